@@ -1,0 +1,147 @@
+"""Lenia field-update benchmark on one MI355X (the LifeBench measurement,
+Apps/LifeBench/main.cpp:43-177, for the Lenia hot path).
+
+Workload (the headline metric's configuration): FFT-form Lenia, 1024x512 FP32
+toroidal field, three ring kernels R=24 with shell peaks [1], [1,.5],
+[.5,1,.25], weights .5/.3/.2, growth (mu, sigma) (.15,.015) (.26,.036)
+(.12,.012), dt .1, seeded 256x256 patch init.  A step is one full field
+update; the field stays resident in HBM.
+
+Prints one JSON line:  value = Mcell-steps/s = W*H*steps / GPU seconds.
+  roofline     dominant kernel: algorithmic bytes per launch / its mean
+               launch time (HIP events on the library's stream)
+  cpu_baseline the FP64 oracle's FFT step on the host cores, bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "life-simulation_amd"), os.path.join(ROOT, "tests"),
+          os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def fft_bytes_per_cell(W: int, K: int) -> dict:
+    """Algorithmic HBM bytes per cell-step of each FFT pass (fp32 state,
+    complex64 spectra, real fp32 kernel spectra, h = (W/2+1)/W):
+      column pass: read F (8h) + read K spectra (4hK) + write K spectra (8hK)
+      row pass:    read K spectra (8hK) + read/write state (8) + write F (8h)
+    """
+    h = (W // 2 + 1) / W
+    cols = 8 * h + 12 * h * K
+    rows = 8 * h * K + 8 + 8 * h
+    return {"cols": cols, "rows": rows, "step": cols + rows}
+
+
+def cpu_baseline(desc: dict, budget_s: float = 12.0) -> dict:
+    import numpy as np
+
+    import oracle as O
+
+    threads = O.max_threads()
+    W, H = desc["width"], desc["height"]
+    m = O.module_seed(7777)
+    x0, y0, pw, ph = desc["initRect"]
+    A = O.init_patch(W, H, m, x0, y0, pw, ph)
+    A = O.step(desc, A, use_fft=True)  # warm-up step
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        A = O.step(desc, A, use_fft=True)
+        n += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(W * H * n / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"FP64 oracle FFT step (oracle/lenia_oracle.c), {n} steps of the same "
+                      f"{W}x{H} K={len(desc['kernels'])} world after 1 warm-up step, "
+                      f"OpenMP {threads} threads, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(worlds shard across GPUs with no communication; see the sharding notes)",
+              file=sys.stderr)
+        return 2
+
+    import lenia
+    from lenia_cases import config1
+
+    desc = config1()
+    W, H, K = desc["width"], desc["height"], len(desc["kernels"])
+    cells = W * H
+    with lenia.Context(0) as ctx, lenia.World(ctx, desc) as w:
+        w.reset(7777)
+        w.frame(0, 1)
+        w.step(args.warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        w.step(args.steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        gpu_ms = ctx.last_gpu_ms()
+        # per-pass durations, events between kernels on the library stream
+        passes = w.pass_times(min(args.steps, 500))
+        tiles = w.resolved()
+        A = w.read()
+    sec = gpu_ms / 1e3
+    value = cells * args.steps / sec / 1e6
+    bpc = fft_bytes_per_cell(W, K)
+    names = ["cols", "rows"]
+    per = {names[i]: passes[i] for i in range(len(passes))}
+    dom = max(per, key=per.get)
+    achieved = bpc[dom] * cells / (per[dom] * 1e-3) / 1e9
+    step_gbs = bpc["step"] * cells * args.steps / sec / 1e9
+    out = {
+        "metric": "Mcell-steps/s, Lenia 3-kernel R=24 FFT 1024x512; batched & adjoint vs plain",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(gpu_ms / args.steps, 5),
+        "wall_ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 256x256 uniform patch (rand01 PCG hash), scene seed 7777",
+        "config": {"workload": "lenia-fft-1024x512-K3-R24", "width": W, "height": H,
+                   "kernels": K, "radius": 24, "growthMode": 0, "dt": 0.1,
+                   "fftPlanW": tiles["fftPlanW"], "fftPlanH": tiles["fftPlanH"],
+                   "rowPairsPerTile": tiles["rowPairsPerTile"],
+                   "colsPerTile": tiles["colsPerTile"]},
+        "passes_ms": {k: round(v, 5) for k, v in per.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_fft_rows" if dom == "rows" else "k_fft_cols",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_cell": round(bpc[dom], 3),
+                     "step_achieved_gbs": round(step_gbs, 1),
+                     "step_bytes_per_cell": round(bpc["step"], 3)},
+        "state_mean": float(A.mean()),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, args.cpu_seconds)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,167 @@
+/*
+ * lenia_mi355x.h -- C ABI of the MI355X-native Lenia field update.
+ *
+ * The library replaces the GPU side of LifeSimSuite's Lenia module: the
+ * module lifecycle (Modules/FieldModules/Lenia.{h,cpp}), its compute passes
+ * (Shaders/Field/Lenia.metal, Shaders/Common/FFT.metal) and the allocator and
+ * timing layer they sit on (LifeCore/Metal/ResourcePool.mm, GPUTimer.mm).
+ * Each entry point names the reference interface it stands in for.
+ *
+ * Conventions: C linkage, plain pointers and sizes, no exceptions.  Every
+ * function returns LENIA_OK (0) or a negative LENIA_ERR_* code and leaves a
+ * message for lenia_last_error() (thread-local).  A context owns one HIP
+ * stream on one device; all work of its worlds and batches is ordered on that
+ * stream.  Device memory is owned by the library; host buffers by the caller.
+ */
+#ifndef LENIA_MI355X_H
+#define LENIA_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LENIA_OK 0
+#define LENIA_ERR_INVALID_ARG -1 /* bad descriptor, size or pointer */
+#define LENIA_ERR_UNSUPPORTED -2 /* valid request this build cannot serve */
+#define LENIA_ERR_DEVICE -3      /* a HIP call failed */
+#define LENIA_ERR_OUT_OF_MEMORY -4
+#define LENIA_ERR_STATE -5       /* call not valid in the object's state */
+
+/* convolution forms ("convMode" of the scene JSON, Lenia.cpp:213-214, plus
+ * the MFMA forms this library adds) */
+#define LENIA_CONV_DIRECT 0        /* FP32 vector direct convolution */
+#define LENIA_CONV_FFT 1           /* FP32 FFT convolution */
+#define LENIA_CONV_DIRECT_BF16 2   /* BF16 MFMA banded contraction, FP32 accumulate */
+#define LENIA_CONV_DIRECT_BF16X3 3 /* split-BF16 (hi+lo) compensated MFMA contraction */
+
+/* kernel shapes: which reference builder a kernel follows */
+#define LENIA_SHAPE_SHELL 0 /* buildKernelTexture (Lenia.cpp:410-436): one shell at shellMu */
+#define LENIA_SHAPE_RING 1  /* buildRingKernelImage (Lenia.cpp:95-125): betas rings, centre 0.5 */
+
+/* kernel cores */
+#define LENIA_CORE_GAUSSIAN 0 /* exp(-((r-c)/s)^2/2), the reference's core */
+#define LENIA_CORE_EXP 1      /* exp(4 - 1/(r(1-r))) exponential bump */
+
+/* init modes */
+#define LENIA_INIT_PATCH 0 /* seeded uniform[0,1) noise in a rectangle, 0 elsewhere */
+#define LENIA_INIT_NOISE 1 /* leniaInit value-noise soup (Lenia.metal:43-61) */
+#define LENIA_INIT_NONE 2  /* keep the current state (set with lenia_world_write_state) */
+
+/* one entry of the scene "kernels" array (Lenia.h:99-105) with the radius
+ * already resolved (radius * radiusScale) */
+typedef struct lenia_kernel_desc {
+    uint32_t shape;   /* LENIA_SHAPE_* */
+    uint32_t core;    /* LENIA_CORE_* */
+    float radius;     /* R_k in cells */
+    float shellMu;    /* kernelShellMu (SHELL shape) */
+    float shellSigma; /* kernelShellSigma */
+    uint32_t nBetas;  /* 1..4 */
+    float betas[4];   /* shell peak heights (RING shape) */
+    float mu;         /* growth centre */
+    float sigma;      /* growth width */
+    float weight;     /* mixing weight h_k */
+} lenia_kernel_desc;
+
+/* one world: the Lenia scene-JSON surface resolved to numbers
+ * (LeniaParams Lenia.h:77-94, LeniaMultiParams Lenia.h:111-129) */
+typedef struct lenia_world_desc {
+    uint32_t width, height; /* simWidth / simHeight */
+    uint32_t convMode;      /* LENIA_CONV_* */
+    uint32_t growthMode;    /* 0: A + dt*sum w(2b-1); 1: asymptotic */
+    float dt;
+    float noiseAmount;
+    float muJitter;
+    float injectAmount;
+    uint32_t injectCount;
+    uint32_t radius; /* module radius; sizes the inject blobs */
+    uint32_t nKernels; /* 1..4 */
+    lenia_kernel_desc kernels[4];
+    /* init applied by the first frame after a reset (needsInit_) */
+    uint32_t initMode; /* LENIA_INIT_* */
+    uint32_t initX0, initY0, initW, initH; /* patch rectangle; initW or initH 0 => whole field */
+    float initCoverage, initScale;         /* value-noise init */
+    /* FFT plan override per axis: radices packed 4 bits each, first stage in
+     * the lowest nibble, 0 = library planner.  Tile overrides: 0 = autotuned */
+    uint32_t fftPlanW, fftPlanH;
+    uint32_t rowPairsPerTile, colsPerTile;
+} lenia_world_desc;
+
+typedef struct lenia_ctx lenia_ctx;
+typedef struct lenia_world lenia_world;
+typedef struct lenia_batch lenia_batch;
+
+/* message of the last failing call on this thread ("" if none) */
+const char *lenia_last_error(void);
+/* library version string */
+const char *lenia_version(void);
+
+/* ---- context: replaces MetalContext::create + ResourcePool + CommandGraph
+ * construction (LifeCore/Sim/SceneRunner.cpp:31-35, MetalContext.mm:91-108) */
+int lenia_ctx_create(int device, lenia_ctx **out);
+int lenia_ctx_destroy(lenia_ctx *ctx);
+/* waits for all queued work; reports a device error raised by it
+ * (CommandGraph::endFrame(wait) + lastFrameHadError, CommandGraph.mm:69-90) */
+int lenia_ctx_sync(lenia_ctx *ctx);
+/* the context's hipStream_t, for callers that order their own work with it */
+int lenia_ctx_stream(lenia_ctx *ctx, void **stream);
+/* live device bytes and allocations (ResourcePool accounting, ResourcePool.h:78-81) */
+int lenia_ctx_memory(lenia_ctx *ctx, uint64_t *liveBytes, uint32_t *liveAllocs);
+/* GPU time in ms of the last lenia_world_step / lenia_batch_step call
+ * (GPUTimer, LifeCore/Metal/GPUTimer.mm:9-94); waits for it */
+int lenia_ctx_last_gpu_ms(lenia_ctx *ctx, double *ms);
+
+/* ---- world: replaces LeniaModule::setup (Lenia.cpp:208-408): allocates the
+ * field buffers, builds the kernels and, for the FFT form, their spectra */
+int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, lenia_world **out);
+int lenia_world_destroy(lenia_world *w);
+/* LeniaModule::reset (Lenia.cpp:448-451): seed_ = deriveSeed(sceneSeed,'LEN1');
+ * the next frame re-initialises the field */
+int lenia_world_reset(lenia_world *w, uint32_t sceneSeed);
+/* LeniaModule::encode (Lenia.cpp:455-653): init if needed, then `substeps`
+ * steps of frame `frameIndex` (step s uses seed pcg(seed_ ^ (frame*197+s))) */
+int lenia_world_frame(lenia_world *w, uint32_t frameIndex, uint32_t substeps);
+/* continues the frame clock: n more frames of one substep each (the
+ * benchmark loop, Apps/LifeBench/main.cpp:112-133) */
+int lenia_world_step(lenia_world *w, uint32_t nSteps);
+/* updates the growth parameters (mu, sigma, weight, dt, growthMode, noise,
+ * jitter, inject) and, if the kernel shapes changed, the kernels and spectra.
+ * Sizes and convMode must be unchanged.  (The per-frame ParameterBus reads of
+ * Lenia.cpp:456-464 and the init-time kernel rebuild of :497-545.) */
+int lenia_world_set_params(lenia_world *w, const lenia_world_desc *desc);
+/* field readback / upload, width*height floats row-major
+ * (FieldModule::outputField, SimulationModule.h:132-135; ResourcePool::uploadTexture,
+ * ResourcePool.mm:175-191).  Uploading clears the pending init. */
+int lenia_world_read_state(lenia_world *w, float *host, size_t count);
+int lenia_world_write_state(lenia_world *w, const float *host, size_t count);
+/* device pointer of the current state buffer (valid until the next step) */
+int lenia_world_state_device(lenia_world *w, float **dev);
+/* potential u_k = K_k (x) A of the current state (the potential_k textures of
+ * Lenia.cpp:611-616), width*height floats */
+int lenia_world_potential(lenia_world *w, uint32_t k, float *host, size_t count);
+/* runs nSteps steps with GPU timestamps between passes and writes the mean
+ * duration in ms of each pass (FFT form: [column pass, row pass]; direct
+ * forms: [step]) -- the per-pass GPUTimer figures LifeBench reports
+ * (LifeCore/Metal/GPUTimer.mm:28-94, Apps/LifeBench/main.cpp:112-133) */
+int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs, uint32_t maxPasses,
+                           uint32_t *nPasses);
+/* resolved descriptor (tile shapes and FFT plans filled in) */
+int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
+
+/* ---- batch: many worlds stepped together (replaces the one-variant-at-a-time
+ * loop of Apps/LifePresetLab/main.cpp:718-806).  Worlds keep their own
+ * buffers; a batch holds a world table and launch tiles over all of them.  All
+ * worlds of a batch must use the same convMode.  Results are bit-identical to
+ * stepping each world alone. */
+int lenia_batch_create(lenia_ctx *ctx, lenia_world *const *worlds, uint32_t n, lenia_batch **out);
+int lenia_batch_destroy(lenia_batch *b);
+/* nSteps steps of every world; graphSteps > 0 captures that many steps in a
+ * hipGraph once and replays it (nSteps must then be a multiple of graphSteps) */
+int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+// Internal host-side structures behind the opaque C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+#include <vector>
+
+#include "../../include/lenia_mi355x.h"
+#include "lenia_types.h"
+
+struct lenia_ctx {
+    int device = 0;
+    int numCUs = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    uint64_t liveBytes = 0;
+    uint32_t liveAllocs = 0;
+    std::vector<std::pair<void *, size_t>> allocSizes;
+};
+
+struct lenia_world {
+    lenia_ctx *ctx = nullptr;
+    lenia_world_desc desc{};
+    bool isFft = false, isMfma = false;
+    lenia::GrowthParams g{};
+    uint32_t moduleSeed = 0;
+    bool needsInit = true;
+    bool specValid = false;   // F holds the spectrum of the current state
+    uint64_t hostSteps = 0;   // steps launched since creation (direct parity)
+    uint32_t frameNext = 0;   // frame index the next lenia_world_step starts at
+    // FFT form
+    lenia::FftPlan planW{}, planH{};
+    int P = 1, C = 1;
+    float *F = nullptr, *G = nullptr, *S = nullptr, *twW = nullptr, *twH = nullptr;
+    // direct form
+    int half = 0;
+    float *winF = nullptr;
+    size_t winBytes = 0;
+    // both
+    float *A0 = nullptr, *A1 = nullptr;
+    float *scratchOut = nullptr;
+    lenia::StepClock *clock = nullptr;
+    uint32_t *done = nullptr;
+    void *devDesc = nullptr;
+    lenia::FftWorldDev fdev{};
+    lenia::DirectWorldDev ddev{};
+    int2 *rowTiles = nullptr, *colTiles = nullptr;
+    int nRowTiles = 0, nColTiles = 0;
+    size_t ldsRows = 0, ldsCols = 0;
+
+    uint32_t parity() const { return (uint32_t)(hostSteps & 1u); }
+    float *current_state() const { return (isFft || parity() == 0) ? A0 : A1; }
+};
+
+namespace lenia {
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+int dev_alloc(lenia_ctx *ctx, void **p, size_t bytes);
+void dev_free(lenia_ctx *ctx, void *p);
+
+double kernel_raw(const lenia_kernel_desc &k, int dx, int dy);
+int kernel_half(const lenia_kernel_desc &k);
+bool kernel_window(const lenia_kernel_desc &k, int h, std::vector<double> &out);
+
+bool plan_default(int n, FftPlan &p);
+bool plan_unpack(int n, uint32_t packed, FftPlan &p);
+uint32_t plan_pack(const FftPlan &p);
+bool plan_fits(const FftPlan &p, int nt, int T, int maxv);
+
+int world_upload_desc(lenia_world *w);
+int world_launch_steps(lenia_world *w, uint32_t n);
+
+uint32_t pcg_hash_host(uint32_t v);
+uint32_t module_seed(uint32_t sceneSeed);
+
+}  // namespace lenia

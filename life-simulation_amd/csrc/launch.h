@@ -1,0 +1,39 @@
+// Host-side launchers of the HIP kernels (defined in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lenia_types.h"
+
+namespace lenia {
+
+// FFT path (k_fft.hip).  mode 0: forward row FFT of the state; 1: full step
+// row pass; 2: potential of kernel kSel into `out`.
+hipError_t launch_fft_rows(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                           int grid, size_t lds, uint32_t *done, int nWorlds, int kSel,
+                           float *out, hipStream_t s);
+hipError_t launch_fft_cols(const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
+                           size_t lds, hipStream_t s);
+hipError_t launch_twiddles(float *tw, int N, hipStream_t s);
+hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double *Cw, float *S,
+                                  hipStream_t s);
+int fft_threads();
+int fft_max_values();
+
+// direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.
+hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
+                         int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                         hipStream_t s);
+void direct_tile_shape(int *tx, int *ty);
+
+// init and clock (k_init.hip)
+hipError_t launch_init_patch(float *A, int W, int H, uint32_t seed, int x0, int y0, int pw,
+                             int ph, hipStream_t s);
+hipError_t launch_init_noise(float *A, int W, int H, uint32_t seed, float coverage, float scale,
+                             hipStream_t s);
+hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t substeps,
+                            uint32_t parity, hipStream_t s);
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+
+}  // namespace lenia

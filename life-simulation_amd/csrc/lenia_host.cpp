@@ -1,0 +1,884 @@
+// Host side of the C ABI: contexts, worlds, batches, kernel building, FFT
+// planning and launch orchestration.  Mirrors LeniaModule's lifecycle
+// (Modules/FieldModules/Lenia.cpp:208-653) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lenia_mi355x.h"
+#include "host_internal.h"
+#include "launch.h"
+
+using namespace lenia;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+int lenia::fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int lenia::hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? LENIA_ERR_OUT_OF_MEMORY : LENIA_ERR_DEVICE,
+                "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                       \
+    do {                                          \
+        hipError_t e_ = (expr);                   \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+extern "C" const char *lenia_last_error(void) { return g_err.c_str(); }
+extern "C" const char *lenia_version(void) { return "lenia-mi355x 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------------ device memory
+int lenia::dev_alloc(lenia_ctx *ctx, void **p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return LENIA_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc");
+    }
+    ctx->liveBytes += bytes;
+    ctx->liveAllocs += 1;
+    ctx->allocSizes.push_back({*p, bytes});
+    return LENIA_OK;
+}
+
+void lenia::dev_free(lenia_ctx *ctx, void *p) {
+    if (!p) return;
+    for (size_t i = 0; i < ctx->allocSizes.size(); ++i) {
+        if (ctx->allocSizes[i].first == p) {
+            ctx->liveBytes -= ctx->allocSizes[i].second;
+            ctx->liveAllocs -= 1;
+            ctx->allocSizes.erase(ctx->allocSizes.begin() + i);
+            break;
+        }
+    }
+    (void)hipFree(p);
+}
+
+// ------------------------------------------------------------------ context
+extern "C" int lenia_ctx_create(int device, lenia_ctx **out) {
+    if (!out) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(LENIA_ERR_UNSUPPORTED, "lenia_ctx_create: device is %s, this build is gfx950 only",
+                    prop.gcnArchName);
+    lenia_ctx *c = new lenia_ctx();
+    c->device = device;
+    c->numCUs = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipStreamCreate");
+    }
+    (void)hipEventCreate(&c->ev0);
+    (void)hipEventCreate(&c->ev1);
+    *out = c;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_ctx_destroy(lenia_ctx *ctx) {
+    if (!ctx) return LENIA_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipEventDestroy(ctx->ev0);
+    (void)hipEventDestroy(ctx->ev1);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_ctx_sync(lenia_ctx *ctx) {
+    if (!ctx) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_sync: NULL context");
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "stream synchronize");
+    return LENIA_OK;
+}
+
+extern "C" int lenia_ctx_stream(lenia_ctx *ctx, void **stream) {
+    if (!ctx || !stream) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_stream: NULL argument");
+    *stream = (void *)ctx->stream;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_ctx_memory(lenia_ctx *ctx, uint64_t *liveBytes, uint32_t *liveAllocs) {
+    if (!ctx) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_memory: NULL context");
+    if (liveBytes) *liveBytes = ctx->liveBytes;
+    if (liveAllocs) *liveAllocs = ctx->liveAllocs;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_ctx_last_gpu_ms(lenia_ctx *ctx, double *ms) {
+    if (!ctx || !ms) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_last_gpu_ms: NULL argument");
+    if (!ctx->timed) return fail(LENIA_ERR_STATE, "no timed step has run on this context");
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    *ms = f;
+    return LENIA_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+// raw (unnormalised) kernel value at integer offset (dx, dy): the two reference
+// builders, buildKernelTexture (Lenia.cpp:419-431) and buildRingKernelImage
+// (Lenia.cpp:103-117), with an optional exponential-bump core.  Evaluated in
+// double, rounded once to float after normalisation.
+static double core_of(uint32_t core, double x, double centre, double sigma) {
+    if (core == LENIA_CORE_EXP) {
+        if (x <= 0.0 || x >= 1.0) return 0.0;
+        return std::exp(4.0 - 1.0 / (x * (1.0 - x)));
+    }
+    double t = (x - centre) / sigma;
+    return std::exp(-0.5 * t * t);
+}
+
+double lenia::kernel_raw(const lenia_kernel_desc &k, int dx, int dy) {
+    double rr = std::sqrt(double(dx) * dx + double(dy) * dy);
+    const double R = k.radius;
+    if (k.shape == LENIA_SHAPE_SHELL) {
+        double r = rr / R;
+        if (!(r > 0.0 && r <= 1.0)) return 0.0;
+        return core_of(k.core, r, k.shellMu, k.shellSigma);
+    }
+    if (!(rr > 0.0 && rr <= R)) return 0.0;
+    const int B = (int)k.nBetas;
+    double u = (rr / R) * B;
+    int i = std::min((int)u, B - 1);
+    double frac = u - i;
+    if (k.core == LENIA_CORE_EXP) return k.betas[i] * core_of(LENIA_CORE_EXP, frac, 0.5, 1.0);
+    double t = (frac - 0.5) / ((double)k.shellSigma * B);
+    return k.betas[i] * std::exp(-0.5 * t * t);
+}
+
+int lenia::kernel_half(const lenia_kernel_desc &k) { return (int)std::ceil((double)k.radius - 1e-9); }
+
+// (2h+1)^2 centred window normalised to sum 1 (double)
+bool lenia::kernel_window(const lenia_kernel_desc &k, int h, std::vector<double> &out) {
+    const int n = 2 * h + 1;
+    out.assign((size_t)n * n, 0.0);
+    double sum = 0.0;
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) {
+            double w = kernel_raw(k, x - h, y - h);
+            out[(size_t)y * n + x] = w;
+            sum += w;
+        }
+    if (!(sum > 0.0)) return false;
+    for (auto &w : out) w /= sum;
+    return true;
+}
+
+// ------------------------------------------------------------------ FFT plans
+static bool factor235(int n, int &a, int &b, int &c) {
+    a = b = c = 0;
+    while (n % 2 == 0) { n /= 2; ++a; }
+    while (n % 3 == 0) { n /= 3; ++b; }
+    while (n % 5 == 0) { n /= 5; ++c; }
+    return n == 1;
+}
+
+bool lenia::plan_default(int n, FftPlan &p) {
+    int a, b, c;
+    if (n < 2 || !factor235(n, a, b, c)) return false;
+    std::vector<int> r;
+    int a8 = a / 3, rem = a % 3;
+    if (rem == 1 && a8 > 0) {
+        a8 -= 1;
+        for (int i = 0; i < a8; ++i) r.push_back(8);
+        r.push_back(4);
+        r.push_back(4);
+    } else {
+        for (int i = 0; i < a8; ++i) r.push_back(8);
+        if (rem == 2) r.push_back(4);
+        if (rem == 1) r.push_back(2);
+    }
+    for (int i = 0; i < b; ++i) r.push_back(3);
+    for (int i = 0; i < c; ++i) r.push_back(5);
+    if ((int)r.size() > kMaxStages) return false;
+    p.n = n;
+    p.nStages = (int)r.size();
+    for (int i = 0; i < kMaxStages; ++i) p.radix[i] = i < p.nStages ? r[i] : 0;
+    return true;
+}
+
+bool lenia::plan_unpack(int n, uint32_t packed, FftPlan &p) {
+    int prod = 1, s = 0;
+    for (; s < kMaxStages; ++s) {
+        int r = (packed >> (4 * s)) & 0xF;
+        if (r == 0) break;
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return false;
+        p.radix[s] = r;
+        prod *= r;
+    }
+    if (prod != n || s == 0) return false;
+    p.n = n;
+    p.nStages = s;
+    for (int i = s; i < kMaxStages; ++i) p.radix[i] = 0;
+    return true;
+}
+
+uint32_t lenia::plan_pack(const FftPlan &p) {
+    uint32_t v = 0;
+    for (int s = 0; s < p.nStages; ++s) v |= (uint32_t)p.radix[s] << (4 * s);
+    return v;
+}
+
+// can one workgroup of T threads run nt transforms of this plan with at most
+// maxv values per thread per stage?
+bool lenia::plan_fits(const FftPlan &p, int nt, int T, int maxv) {
+    for (int s = 0; s < p.nStages; ++s) {
+        int R = p.radix[s];
+        long nb = (long)nt * p.n / R;
+        long perThread = (nb + T - 1) / T;
+        if (perThread * R > maxv || perThread > maxv / R) return false;
+    }
+    return (long)nt * p.n <= (long)T * maxv;
+}
+
+// ------------------------------------------------------------------ descriptor checks
+static int check_desc(const lenia_world_desc *d) {
+    if (!d) return fail(LENIA_ERR_INVALID_ARG, "world descriptor is NULL");
+    if (d->width < 1 || d->height < 1 || d->width > 16384 || d->height > 16384)
+        return fail(LENIA_ERR_INVALID_ARG, "field size %ux%u out of range", d->width, d->height);
+    if (d->nKernels < 1 || d->nKernels > 4)
+        return fail(LENIA_ERR_INVALID_ARG, "nKernels must be 1..4 (got %u)", d->nKernels);
+    if (d->convMode > LENIA_CONV_DIRECT_BF16X3)
+        return fail(LENIA_ERR_INVALID_ARG, "unknown convMode %u", d->convMode);
+    if (d->growthMode > 1) return fail(LENIA_ERR_INVALID_ARG, "growthMode must be 0 or 1");
+    if (!(d->dt >= 0.0f) || !std::isfinite(d->dt)) return fail(LENIA_ERR_INVALID_ARG, "dt must be finite and >= 0");
+    if (d->initMode > LENIA_INIT_NONE) return fail(LENIA_ERR_INVALID_ARG, "unknown initMode");
+    for (uint32_t k = 0; k < d->nKernels; ++k) {
+        const lenia_kernel_desc &kd = d->kernels[k];
+        if (kd.shape > LENIA_SHAPE_RING || kd.core > LENIA_CORE_EXP)
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u: unknown shape/core", k);
+        if (!(kd.radius >= 1.0f) || kd.radius > 256.0f)
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u: radius %g out of range [1,256]", k, kd.radius);
+        if (kd.nBetas < 1 || kd.nBetas > 4)
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u: nBetas must be 1..4", k);
+        if (!(kd.sigma > 0.0f) || !std::isfinite(kd.mu))
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u: growth sigma must be > 0", k);
+        if (kd.core == LENIA_CORE_GAUSSIAN && !(kd.shellSigma > 0.0f))
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u: shellSigma must be > 0", k);
+        int h = kernel_half(kd);
+        if (d->convMode != LENIA_CONV_FFT && h > 45)
+            return fail(LENIA_ERR_UNSUPPORTED,
+                        "kernel %u: direct forms stage a (16+2h)x(64+2h) LDS tile and support "
+                        "h <= 45 (got %d); use the FFT form", k, h);
+        if (2 * h + 1 > (int)std::min(d->width, d->height))
+            return fail(LENIA_ERR_INVALID_ARG,
+                        "kernel %u: window 2*%d+1 exceeds the field (%ux%u); the torus would "
+                        "wrap onto itself", k, h, d->width, d->height);
+    }
+    return LENIA_OK;
+}
+
+static GrowthParams growth_of(const lenia_world_desc *d, uint32_t moduleSeed) {
+    GrowthParams g;
+    std::memset(&g, 0, sizeof(g));
+    g.dt = d->dt;
+    g.nKernels = d->nKernels;
+    g.growthMode = d->growthMode;
+    g.noiseAmount = d->noiseAmount;
+    g.muJitter = d->muJitter;
+    g.injectAmount = d->injectAmount;
+    g.injectCount = d->injectCount;
+    g.radius = d->radius;
+    g.moduleSeed = moduleSeed;
+    float sw = 0.0f;
+    for (int k = 0; k < kMaxKernels; ++k) {
+        bool on = k < (int)d->nKernels;
+        g.mu[k] = on ? d->kernels[k].mu : 0.15f;
+        g.sigma[k] = on ? d->kernels[k].sigma : 0.017f;
+        g.weight[k] = on ? d->kernels[k].weight : 0.0f;
+        if (on) sw += g.weight[k];  // same accumulation order as leniaGrowthMulti
+    }
+    g.sumWeights = sw;
+    return g;
+}
+
+// ------------------------------------------------------------------ world setup
+static int build_kernels_direct(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    const lenia_world_desc &d = w->desc;
+    int h = 0;
+    for (uint32_t k = 0; k < d.nKernels; ++k) h = std::max(h, kernel_half(d.kernels[k]));
+    w->half = h;
+    const int n = 2 * h + 1;
+    std::vector<float> all((size_t)d.nKernels * n * n, 0.0f);
+    std::vector<double> win;
+    for (uint32_t k = 0; k < d.nKernels; ++k) {
+        if (!kernel_window(d.kernels[k], h, win))
+            return fail(LENIA_ERR_INVALID_ARG, "kernel %u has zero mass", k);
+        for (size_t i = 0; i < win.size(); ++i) all[(size_t)k * n * n + i] = (float)win[i];
+    }
+    size_t bytes = all.size() * sizeof(float);
+    if (!w->winF || w->winBytes != bytes) {
+        dev_free(ctx, w->winF);
+        int rc = dev_alloc(ctx, (void **)&w->winF, bytes);
+        if (rc) return rc;
+        w->winBytes = bytes;
+    }
+    HIP_TRY(hipMemcpyAsync(w->winF, all.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+            "upload kernel windows");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "upload kernel windows");
+    return LENIA_OK;
+}
+
+static int build_kernels_fft(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    const lenia_world_desc &d = w->desc;
+    const int W = d.width, H = d.height, Wh = W / 2 + 1;
+    std::vector<double> win;
+    int hmax = 0;
+    for (uint32_t k = 0; k < d.nKernels; ++k) hmax = std::max(hmax, kernel_half(d.kernels[k]));
+    const int nmax = 2 * hmax + 1;
+    double *dWin = nullptr, *dCw = nullptr;
+    int rc = dev_alloc(ctx, (void **)&dWin, sizeof(double) * nmax * nmax);
+    if (rc) return rc;
+    rc = dev_alloc(ctx, (void **)&dCw, sizeof(double) * nmax * Wh);
+    if (rc) {
+        dev_free(ctx, dWin);
+        return rc;
+    }
+    for (uint32_t k = 0; k < d.nKernels && rc == LENIA_OK; ++k) {
+        int h = kernel_half(d.kernels[k]);
+        if (!kernel_window(d.kernels[k], h, win)) {
+            rc = fail(LENIA_ERR_INVALID_ARG, "kernel %u has zero mass", k);
+            break;
+        }
+        hipError_t e = hipMemcpyAsync(dWin, win.data(), sizeof(double) * win.size(),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_kernel_spectrum(dWin, h, W, H, dCw, w->S + (size_t)k * Wh * H, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "kernel spectrum");
+    }
+    dev_free(ctx, dWin);
+    dev_free(ctx, dCw);
+    return rc;
+}
+
+static void choose_tiles_fft(lenia_world *w, int numCUs) {
+    const lenia_world_desc &d = w->desc;
+    const int W = d.width, H = d.height, Wh = W / 2 + 1;
+    const int T = fft_threads(), mv = fft_max_values();
+    // largest row-pair count that fits, then shrink while the grid is
+    // smaller than the chip (more tiles beat fatter tiles until every CU
+    // has work)
+    int P = d.rowPairsPerTile;
+    if (P == 0) {
+        P = 1;
+        while (plan_fits(w->planW, P * 2, T, mv) && P * 2 <= H / 2) P *= 2;
+        while (P > 1 && (H / 2 + P - 1) / P < numCUs) P /= 2;
+    }
+    int C = d.colsPerTile;
+    if (C == 0) {
+        C = 1;
+        while (plan_fits(w->planH, C * 2, T, mv) && C * 2 <= Wh) C *= 2;
+        while (C > 1 && (Wh + C - 1) / C < numCUs) C /= 2;
+    }
+    w->P = P;
+    w->C = C;
+}
+
+static int validate_tiles_fft(lenia_world *w) {
+    const int T = fft_threads(), mv = fft_max_values();
+    if (w->P < 1 || !plan_fits(w->planW, w->P, T, mv))
+        return fail(LENIA_ERR_INVALID_ARG, "row tile of %d row pairs does not fit a workgroup for W=%d",
+                    w->P, w->desc.width);
+    if (w->C < 1 || !plan_fits(w->planH, w->C, T, mv))
+        return fail(LENIA_ERR_INVALID_ARG, "column tile of %d columns does not fit a workgroup for H=%d",
+                    w->C, w->desc.height);
+    return LENIA_OK;
+}
+
+int lenia::world_upload_desc(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    if (w->isFft) {
+        FftWorldDev f;
+        std::memset(&f, 0, sizeof(f));
+        f.W = w->desc.width;
+        f.H = w->desc.height;
+        f.Wh = f.W / 2 + 1;
+        f.P = w->P;
+        f.C = w->C;
+        f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
+        f.colTiles = (f.Wh + f.C - 1) / f.C;
+        f.planW = w->planW;
+        f.planH = w->planH;
+        f.g = w->g;
+        f.twW = w->twW;
+        f.twH = w->twH;
+        f.A = w->A0;
+        f.F = w->F;
+        f.G = w->G;
+        f.S = w->S;
+        f.clock = w->clock;
+        w->fdev = f;
+        HIP_TRY(hipMemcpyAsync(w->devDesc, &f, sizeof(f), hipMemcpyHostToDevice, ctx->stream),
+                "upload world descriptor");
+        std::vector<int2> rt, ct;
+        for (int i = 0; i < f.rowTiles; ++i) rt.push_back(make_int2(0, i));
+        for (int i = 0; i < f.colTiles; ++i) ct.push_back(make_int2(0, i));
+        HIP_TRY(hipMemcpyAsync(w->rowTiles, rt.data(), rt.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
+        HIP_TRY(hipMemcpyAsync(w->colTiles, ct.data(), ct.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
+        w->nRowTiles = f.rowTiles;
+        w->nColTiles = f.colTiles;
+        w->ldsRows = (size_t)f.P * f.W * sizeof(float2);
+        w->ldsCols = (size_t)f.C * f.H * sizeof(float2);
+    } else {
+        DirectWorldDev dd;
+        std::memset(&dd, 0, sizeof(dd));
+        int tx, ty;
+        direct_tile_shape(&tx, &ty);
+        dd.W = w->desc.width;
+        dd.H = w->desc.height;
+        dd.half = w->half;
+        dd.tilesX = (dd.W + tx - 1) / tx;
+        dd.tilesY = (dd.H + ty - 1) / ty;
+        dd.g = w->g;
+        dd.win = w->winF;
+        dd.A0 = w->A0;
+        dd.A1 = w->A1;
+        dd.clock = w->clock;
+        w->ddev = dd;
+        HIP_TRY(hipMemcpyAsync(w->devDesc, &dd, sizeof(dd), hipMemcpyHostToDevice, ctx->stream),
+                "upload world descriptor");
+        std::vector<int2> t;
+        for (int i = 0; i < dd.tilesX * dd.tilesY; ++i) t.push_back(make_int2(0, i));
+        HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
+        w->nRowTiles = (int)t.size();
+        w->ldsRows = (size_t)(ty + 2 * w->half) * (tx + 2 * w->half) * sizeof(float);
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "upload world descriptor");
+    return LENIA_OK;
+}
+
+static void world_release(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF,
+                    w->clock, w->done, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
+    for (void *p : ptrs) dev_free(ctx, p);
+}
+
+extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, lenia_world **out) {
+    if (!ctx || !out) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_create: NULL argument");
+    *out = nullptr;
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    lenia_world *w = new lenia_world();
+    w->ctx = ctx;
+    w->desc = *desc;
+    w->isFft = desc->convMode == LENIA_CONV_FFT;
+    w->isMfma = desc->convMode == LENIA_CONV_DIRECT_BF16 || desc->convMode == LENIA_CONV_DIRECT_BF16X3;
+    const int W = desc->width, H = desc->height;
+    if (w->isFft) {
+        if (W % 2 || H % 2) {
+            delete w;
+            return fail(LENIA_ERR_UNSUPPORTED, "FFT form needs even width and height (got %dx%d)", W, H);
+        }
+        bool okW = desc->fftPlanW ? plan_unpack(W, desc->fftPlanW, w->planW) : plan_default(W, w->planW);
+        bool okH = desc->fftPlanH ? plan_unpack(H, desc->fftPlanH, w->planH) : plan_default(H, w->planH);
+        if (!okW || !okH) {
+            delete w;
+            return fail(LENIA_ERR_UNSUPPORTED,
+                        "FFT form needs sides of the form 2^a 3^b 5^c with a valid radix plan "
+                        "(got %dx%d)", W, H);
+        }
+        if (!plan_fits(w->planW, 1, fft_threads(), fft_max_values()) ||
+            !plan_fits(w->planH, 1, fft_threads(), fft_max_values())) {
+            delete w;
+            return fail(LENIA_ERR_UNSUPPORTED, "FFT length above 4096 (got %dx%d)", W, H);
+        }
+        choose_tiles_fft(w, ctx->numCUs);
+        rc = validate_tiles_fft(w);
+        if (rc) {
+            delete w;
+            return rc;
+        }
+    }
+    if (w->isMfma) {
+        delete w;
+        return fail(LENIA_ERR_UNSUPPORTED, "MFMA direct forms are not built yet");
+    }
+    w->moduleSeed = 0;
+    w->g = growth_of(desc, w->moduleSeed);
+    const size_t cells = (size_t)W * H;
+    const int Wh = W / 2 + 1;
+    const size_t K = desc->nKernels;
+    rc = dev_alloc(ctx, (void **)&w->A0, cells * sizeof(float));
+    if (!rc && !w->isFft) rc = dev_alloc(ctx, (void **)&w->A1, cells * sizeof(float));
+    if (!rc) rc = dev_alloc(ctx, (void **)&w->clock, sizeof(StepClock));
+    if (!rc) rc = dev_alloc(ctx, (void **)&w->done, sizeof(uint32_t) * 4);
+    if (!rc) rc = dev_alloc(ctx, (void **)&w->devDesc, std::max(sizeof(FftWorldDev), sizeof(DirectWorldDev)));
+    if (!rc) rc = dev_alloc(ctx, (void **)&w->scratchOut, cells * sizeof(float));
+    if (w->isFft) {
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->F, (size_t)Wh * H * sizeof(float2));
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->G, K * Wh * H * sizeof(float2));
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->S, K * Wh * H * sizeof(float));
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->twW, (size_t)W * sizeof(float2));
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->twH, (size_t)H * sizeof(float2));
+        int nr = (H / 2 + w->P - 1) / w->P, nc = (Wh + w->C - 1) / w->C;
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nr * sizeof(int2));
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->colTiles, nc * sizeof(int2));
+    } else {
+        int tx, ty;
+        direct_tile_shape(&tx, &ty);
+        int nt = ((W + tx - 1) / tx) * ((H + ty - 1) / ty);
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nt * sizeof(int2));
+    }
+    if (rc) {
+        world_release(w);
+        delete w;
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(w->done, 0, sizeof(uint32_t) * 4, ctx->stream);
+    if (e == hipSuccess) e = launch_fill(w->A0, cells, 0.0f, ctx->stream);
+    if (e == hipSuccess && w->A1) e = launch_fill(w->A1, cells, 0.0f, ctx->stream);
+    if (e == hipSuccess) e = launch_set_clock(w->clock, 0, 0, 1, 0, ctx->stream);
+    if (e == hipSuccess && w->isFft) {
+        e = launch_twiddles(w->twW, W, ctx->stream);
+        if (e == hipSuccess) e = launch_twiddles(w->twH, H, ctx->stream);
+        if (e == hipSuccess) e = launch_fill(w->G, K * Wh * H * 2, 0.0f, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "world init");
+        world_release(w);
+        delete w;
+        return rc;
+    }
+    rc = w->isFft ? build_kernels_fft(w) : build_kernels_direct(w);
+    if (!rc) rc = world_upload_desc(w);
+    if (rc) {
+        world_release(w);
+        delete w;
+        return rc;
+    }
+    w->needsInit = desc->initMode != LENIA_INIT_NONE;
+    w->specValid = false;
+    w->frameNext = 0;
+    *out = w;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_destroy(lenia_world *w) {
+    if (!w) return LENIA_OK;
+    (void)hipSetDevice(w->ctx->device);
+    (void)hipStreamSynchronize(w->ctx->stream);
+    world_release(w);
+    delete w;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_reset(lenia_world *w, uint32_t sceneSeed) {
+    if (!w) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_reset: NULL world");
+    (void)hipSetDevice(w->ctx->device);
+    w->moduleSeed = module_seed(sceneSeed);
+    w->g.moduleSeed = w->moduleSeed;
+    w->needsInit = w->desc.initMode != LENIA_INIT_NONE;
+    return world_upload_desc(w);
+}
+
+// init pass (the needsInit_ branch of LeniaModule::encode, Lenia.cpp:468-495)
+static int world_apply_init(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    const lenia_world_desc &d = w->desc;
+    const int W = d.width, H = d.height;
+    float *cur = w->current_state();
+    hipError_t e = hipSuccess;
+    if (d.initMode == LENIA_INIT_PATCH) {
+        int x0 = d.initX0, y0 = d.initY0, pw = d.initW, ph = d.initH;
+        if (pw == 0 || ph == 0) {
+            x0 = y0 = 0;
+            pw = W;
+            ph = H;
+        }
+        e = launch_init_patch(cur, W, H, w->moduleSeed, x0, y0, pw, ph, ctx->stream);
+    } else if (d.initMode == LENIA_INIT_NOISE) {
+        e = launch_init_noise(cur, W, H, w->moduleSeed, d.initCoverage, d.initScale, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "init pass");
+    w->needsInit = false;
+    w->specValid = false;
+    return LENIA_OK;
+}
+
+// F must hold the spectrum of the current state before a row+column step
+static int world_ensure_spectrum(lenia_world *w) {
+    if (!w->isFft || w->specValid) return LENIA_OK;
+    lenia_ctx *ctx = w->ctx;
+    hipError_t e = launch_fft_rows(0, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                                   w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "forward row pass");
+    w->specValid = true;
+    return LENIA_OK;
+}
+
+int lenia::world_launch_steps(lenia_world *w, uint32_t n) {
+    lenia_ctx *ctx = w->ctx;
+    int rc = world_ensure_spectrum(w);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i) {
+        hipError_t e;
+        if (w->isFft) {
+            e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles,
+                                w->nColTiles, w->ldsCols, ctx->stream);
+            if (e == hipSuccess)
+                e = launch_fft_rows(1, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                                    w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+        } else {
+            e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                              w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+        }
+        if (e != hipSuccess) return hip_fail(e, "step launch");
+        w->hostSteps += 1;
+    }
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_frame(lenia_world *w, uint32_t frameIndex, uint32_t substeps) {
+    if (!w) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_frame: NULL world");
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (w->needsInit) {
+        int rc = world_apply_init(w);
+        if (rc) return rc;
+    }
+    uint32_t steps = std::max(1u, substeps);
+    hipError_t e = launch_set_clock(w->clock, frameIndex, 0, steps, w->parity(), ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "set clock");
+    w->frameNext = frameIndex + 1;
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    int rc = world_launch_steps(w, steps);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->timed = true;
+    return rc;
+}
+
+extern "C" int lenia_world_step(lenia_world *w, uint32_t nSteps) {
+    if (!w) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_step: NULL world");
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    if (w->needsInit) {
+        int rc = world_apply_init(w);
+        if (rc) return rc;
+        hipError_t e = launch_set_clock(w->clock, w->frameNext, 0, 1, w->parity(), ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "set clock");
+    }
+    w->frameNext += nSteps;
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    int rc = world_launch_steps(w, nSteps);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->timed = true;
+    return rc;
+}
+
+extern "C" int lenia_world_set_params(lenia_world *w, const lenia_world_desc *desc) {
+    if (!w) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_set_params: NULL world");
+    int rc = check_desc(desc);
+    if (rc) return rc;
+    if (desc->width != w->desc.width || desc->height != w->desc.height ||
+        desc->convMode != w->desc.convMode || desc->nKernels != w->desc.nKernels)
+        return fail(LENIA_ERR_INVALID_ARG, "set_params cannot change size, convMode or kernel count");
+    (void)hipSetDevice(w->ctx->device);
+    bool shapes = false;
+    for (uint32_t k = 0; k < desc->nKernels; ++k) {
+        const lenia_kernel_desc &a = desc->kernels[k], &b = w->desc.kernels[k];
+        if (a.shape != b.shape || a.core != b.core || a.radius != b.radius ||
+            a.shellMu != b.shellMu || a.shellSigma != b.shellSigma || a.nBetas != b.nBetas ||
+            std::memcmp(a.betas, b.betas, sizeof(a.betas)) != 0)
+            shapes = true;
+    }
+    if (w->isFft) {
+        for (uint32_t k = 0; k < desc->nKernels; ++k)
+            if (kernel_half(desc->kernels[k]) > std::min(desc->width, desc->height) / 2)
+                return fail(LENIA_ERR_INVALID_ARG, "kernel window too large");
+    } else if (shapes) {
+        int h = 0;
+        for (uint32_t k = 0; k < desc->nKernels; ++k) h = std::max(h, kernel_half(desc->kernels[k]));
+        if (h != w->half)
+            return fail(LENIA_ERR_INVALID_ARG, "set_params cannot change the direct window size");
+    }
+    uint32_t keepPW = w->desc.fftPlanW, keepPH = w->desc.fftPlanH;
+    uint32_t keepP = w->desc.rowPairsPerTile, keepC = w->desc.colsPerTile;
+    w->desc = *desc;
+    w->desc.fftPlanW = keepPW;
+    w->desc.fftPlanH = keepPH;
+    w->desc.rowPairsPerTile = keepP;
+    w->desc.colsPerTile = keepC;
+    w->g = growth_of(desc, w->moduleSeed);
+    if (shapes) {
+        rc = w->isFft ? build_kernels_fft(w) : build_kernels_direct(w);
+        if (rc) return rc;
+    }
+    return world_upload_desc(w);
+}
+
+extern "C" int lenia_world_read_state(lenia_world *w, float *host, size_t count) {
+    if (!w || !host) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_read_state: NULL argument");
+    size_t cells = (size_t)w->desc.width * w->desc.height;
+    if (count != cells)
+        return fail(LENIA_ERR_INVALID_ARG, "read_state: count %zu != width*height %zu", count, cells);
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(hipMemcpyAsync(host, w->current_state(), cells * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream), "read state");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "read state");
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_write_state(lenia_world *w, const float *host, size_t count) {
+    if (!w || !host) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_write_state: NULL argument");
+    size_t cells = (size_t)w->desc.width * w->desc.height;
+    if (count != cells)
+        return fail(LENIA_ERR_INVALID_ARG, "write_state: count %zu != width*height %zu", count, cells);
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(hipMemcpyAsync(w->current_state(), host, cells * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->stream), "write state");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "write state");
+    w->needsInit = false;
+    w->specValid = false;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_state_device(lenia_world *w, float **dev) {
+    if (!w || !dev) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_state_device: NULL argument");
+    *dev = w->current_state();
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, size_t count) {
+    if (!w || !host) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_potential: NULL argument");
+    if (k >= w->desc.nKernels) return fail(LENIA_ERR_INVALID_ARG, "kernel index %u out of range", k);
+    size_t cells = (size_t)w->desc.width * w->desc.height;
+    if (count != cells) return fail(LENIA_ERR_INVALID_ARG, "potential: count != width*height");
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e;
+    if (w->isFft) {
+        int rc = world_ensure_spectrum(w);
+        if (rc) return rc;
+        e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles, w->nColTiles,
+                            w->ldsCols, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_fft_rows(2, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                                w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut,
+                                ctx->stream);
+    } else {
+        e = launch_direct(1, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                          w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "potential launch");
+    HIP_TRY(hipMemcpyAsync(host, w->scratchOut, cells * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream), "read potential");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "read potential");
+    return LENIA_OK;
+}
+
+extern "C" int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out) {
+    if (!w || !out) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_get_desc: NULL argument");
+    *out = w->desc;
+    if (w->isFft) {
+        out->fftPlanW = plan_pack(w->planW);
+        out->fftPlanH = plan_pack(w->planH);
+        out->rowPairsPerTile = w->P;
+        out->colsPerTile = w->C;
+    }
+    return LENIA_OK;
+}
+
+// pcgHash / deriveSeed (LifeCore/Math/Random.h:34-44) on the host
+uint32_t lenia::pcg_hash_host(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+uint32_t lenia::module_seed(uint32_t sceneSeed) {
+    return pcg_hash_host(sceneSeed ^ pcg_hash_host(0x4C454E31u));  // "LEN1"
+}
+
+// per-pass GPU timing of nSteps steps (GPUTimer per-pass timestamps,
+// LifeCore/Metal/GPUTimer.mm:28-94, averaged the way LifeBench reports them,
+// Apps/LifeBench/main.cpp:112-133).  Events are recorded on the context stream
+// between passes, so each figure is one kernel's duration.
+extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs,
+                                      uint32_t maxPasses, uint32_t *nPasses) {
+    if (!w || !outMs || !nPasses || nSteps == 0)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_world_pass_times: bad argument");
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    const uint32_t np = w->isFft ? 2u : 1u;
+    if (maxPasses < np) return fail(LENIA_ERR_INVALID_ARG, "pass_times: need room for %u passes", np);
+    if (w->needsInit) {
+        int rc = world_apply_init(w);
+        if (rc) return rc;
+    }
+    int rc = world_ensure_spectrum(w);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev((size_t)nSteps * np + 1);
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+    hipError_t e = hipEventRecord(ev[0], ctx->stream);
+    size_t ei = 1;
+    for (uint32_t i = 0; i < nSteps && e == hipSuccess; ++i) {
+        if (w->isFft) {
+            e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles,
+                                w->nColTiles, w->ldsCols, ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(ev[ei++], ctx->stream);
+            if (e == hipSuccess)
+                e = launch_fft_rows(1, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                                    w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+        } else {
+            e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                              w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+        }
+        if (e == hipSuccess) e = hipEventRecord(ev[ei++], ctx->stream);
+        if (e == hipSuccess) w->hostSteps += 1;
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(ev[ei - 1]);
+    for (uint32_t p = 0; p < np; ++p) outMs[p] = 0.0;
+    if (e == hipSuccess) {
+        for (uint32_t i = 0; i < nSteps; ++i)
+            for (uint32_t p = 0; p < np; ++p) {
+                float ms = 0.f;
+                hipError_t e2 = hipEventElapsedTime(&ms, ev[(size_t)i * np + p], ev[(size_t)i * np + p + 1]);
+                if (e2 != hipSuccess) e = e2;
+                outMs[p] += ms / nSteps;
+            }
+    }
+    for (auto &x : ev) (void)hipEventDestroy(x);
+    w->frameNext += nSteps;
+    if (e != hipSuccess) return hip_fail(e, "pass timing");
+    *nPasses = np;
+    return LENIA_OK;
+}

@@ -1,0 +1,184 @@
+"""ctypes binding of liblenia_mi355x.so (the C ABI in include/lenia_mi355x.h).
+
+This module only loads the library and mirrors its structs; it never falls
+back to anything else.  If the shared library is missing, every entry point
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "liblenia_mi355x.so")
+
+OK = 0
+ERR_INVALID_ARG = -1
+ERR_UNSUPPORTED = -2
+ERR_DEVICE = -3
+ERR_OUT_OF_MEMORY = -4
+ERR_STATE = -5
+
+CONV_DIRECT, CONV_FFT, CONV_DIRECT_BF16, CONV_DIRECT_BF16X3 = 0, 1, 2, 3
+SHAPE_SHELL, SHAPE_RING = 0, 1
+CORE_GAUSSIAN, CORE_EXP = 0, 1
+INIT_PATCH, INIT_NOISE, INIT_NONE = 0, 1, 2
+
+# every symbol include/lenia_mi355x.h declares
+EXPORTS = [
+    "lenia_last_error", "lenia_version",
+    "lenia_ctx_create", "lenia_ctx_destroy", "lenia_ctx_sync", "lenia_ctx_stream",
+    "lenia_ctx_memory", "lenia_ctx_last_gpu_ms",
+    "lenia_world_create", "lenia_world_destroy", "lenia_world_reset", "lenia_world_frame",
+    "lenia_world_step", "lenia_world_set_params", "lenia_world_read_state",
+    "lenia_world_write_state", "lenia_world_state_device", "lenia_world_potential",
+    "lenia_world_get_desc", "lenia_world_pass_times",
+    "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
+]
+
+
+class KernelDesc(C.Structure):
+    _fields_ = [
+        ("shape", C.c_uint32),
+        ("core", C.c_uint32),
+        ("radius", C.c_float),
+        ("shellMu", C.c_float),
+        ("shellSigma", C.c_float),
+        ("nBetas", C.c_uint32),
+        ("betas", C.c_float * 4),
+        ("mu", C.c_float),
+        ("sigma", C.c_float),
+        ("weight", C.c_float),
+    ]
+
+
+class WorldDesc(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("convMode", C.c_uint32),
+        ("growthMode", C.c_uint32),
+        ("dt", C.c_float),
+        ("noiseAmount", C.c_float),
+        ("muJitter", C.c_float),
+        ("injectAmount", C.c_float),
+        ("injectCount", C.c_uint32),
+        ("radius", C.c_uint32),
+        ("nKernels", C.c_uint32),
+        ("kernels", KernelDesc * 4),
+        ("initMode", C.c_uint32),
+        ("initX0", C.c_uint32),
+        ("initY0", C.c_uint32),
+        ("initW", C.c_uint32),
+        ("initH", C.c_uint32),
+        ("initCoverage", C.c_float),
+        ("initScale", C.c_float),
+        ("fftPlanW", C.c_uint32),
+        ("fftPlanH", C.c_uint32),
+        ("rowPairsPerTile", C.c_uint32),
+        ("colsPerTile", C.c_uint32),
+    ]
+
+
+class LeniaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lenia error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u32, sz = C.c_void_p, C.c_int, C.c_uint32, C.c_size_t
+    fp = C.POINTER(C.c_float)
+    sig = {
+        "lenia_last_error": (C.c_char_p, []),
+        "lenia_version": (C.c_char_p, []),
+        "lenia_ctx_create": (i32, [i32, C.POINTER(vp)]),
+        "lenia_ctx_destroy": (i32, [vp]),
+        "lenia_ctx_sync": (i32, [vp]),
+        "lenia_ctx_stream": (i32, [vp, C.POINTER(vp)]),
+        "lenia_ctx_memory": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+        "lenia_ctx_last_gpu_ms": (i32, [vp, C.POINTER(C.c_double)]),
+        "lenia_world_create": (i32, [vp, C.POINTER(WorldDesc), C.POINTER(vp)]),
+        "lenia_world_destroy": (i32, [vp]),
+        "lenia_world_reset": (i32, [vp, u32]),
+        "lenia_world_frame": (i32, [vp, u32, u32]),
+        "lenia_world_step": (i32, [vp, u32]),
+        "lenia_world_set_params": (i32, [vp, C.POINTER(WorldDesc)]),
+        "lenia_world_read_state": (i32, [vp, fp, sz]),
+        "lenia_world_write_state": (i32, [vp, fp, sz]),
+        "lenia_world_state_device": (i32, [vp, C.POINTER(vp)]),
+        "lenia_world_potential": (i32, [vp, u32, fp, sz]),
+        "lenia_world_get_desc": (i32, [vp, C.POINTER(WorldDesc)]),
+        "lenia_world_pass_times": (i32, [vp, u32, C.POINTER(C.c_double), u32,
+                                         C.POINTER(C.c_uint32)]),
+        "lenia_batch_create": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
+        "lenia_batch_destroy": (i32, [vp]),
+        "lenia_batch_step": (i32, [vp, u32, u32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise LeniaError(rc, lib().lenia_last_error().decode())
+
+
+def kernel_desc(k: dict) -> KernelDesc:
+    d = KernelDesc()
+    d.shape = int(k.get("shape", SHAPE_RING))
+    d.core = int(k.get("core", CORE_GAUSSIAN))
+    d.radius = float(k["radius"])
+    d.shellMu = float(k.get("shellMu", 0.5))
+    d.shellSigma = float(k.get("shellSigma", 0.15))
+    betas = list(k.get("betas", [1.0]))
+    d.nBetas = len(betas)
+    for i, b in enumerate(betas[:4]):
+        d.betas[i] = float(b)
+    d.mu = float(k.get("mu", 0.15))
+    d.sigma = float(k.get("sigma", 0.017))
+    d.weight = float(k.get("weight", 1.0))
+    return d
+
+
+def world_desc(w: dict) -> WorldDesc:
+    """Builds a WorldDesc from the dict form shared with the oracle."""
+    d = WorldDesc()
+    d.width = int(w["width"])
+    d.height = int(w["height"])
+    d.convMode = int(w.get("convMode", CONV_DIRECT))
+    d.growthMode = int(w.get("growthMode", 0))
+    d.dt = float(w.get("dt", 0.1))
+    d.noiseAmount = float(w.get("noiseAmount", 0.0))
+    d.muJitter = float(w.get("muJitter", 0.0))
+    d.injectAmount = float(w.get("injectAmount", 0.0))
+    d.injectCount = int(w.get("injectCount", 3))
+    d.radius = int(w.get("radius", 13))
+    ks = w["kernels"]
+    d.nKernels = len(ks)
+    for i, k in enumerate(ks[:4]):
+        d.kernels[i] = kernel_desc(k)
+    d.initMode = int(w.get("initMode", INIT_PATCH))
+    x0, y0, pw, ph = w.get("initRect", (0, 0, 0, 0))
+    d.initX0, d.initY0, d.initW, d.initH = int(x0), int(y0), int(pw), int(ph)
+    d.initCoverage = float(w.get("initCoverage", 0.4))
+    d.initScale = float(w.get("initScale", 24.0))
+    d.fftPlanW = int(w.get("fftPlanW", 0))
+    d.fftPlanH = int(w.get("fftPlanH", 0))
+    d.rowPairsPerTile = int(w.get("rowPairsPerTile", 0))
+    d.colsPerTile = int(w.get("colsPerTile", 0))
+    return d
