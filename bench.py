@@ -67,6 +67,51 @@ def cpu_baseline(desc: dict, budget_s: float = 12.0) -> dict:
                       f"OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int = 20) -> dict:
+    """256 uneven worlds stepped in one batch (graph-replayed) vs the same worlds
+    stepped one world per launch sequence, one after another (the PresetLab
+    loop, Apps/LifePresetLab/main.cpp:718-806)."""
+    import lenia
+    from lenia_cases import sweep_worlds
+
+    descs = sweep_worlds(256)
+    worlds = [lenia.World(ctx, d) for d in descs]
+    try:
+        for i, w in enumerate(worlds):
+            w.reset(1000 + i)
+            w.frame(0, 1)
+        cells = sum(d["width"] * d["height"] for d in descs)
+        bytes_step = sum(d["width"] * d["height"] *
+                         fft_bytes_per_cell(d["width"], len(d["kernels"]))["step"] for d in descs)
+        with lenia.Batch(ctx, worlds) as b:
+            b.step(graph_steps, graph_steps)  # warm-up + graph capture
+            ctx.sync()
+            b.step(steps, graph_steps)
+            batch_ms = ctx.last_gpu_ms()
+        # plain: each world alone, one after the other
+        for w in worlds:
+            w.step(2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for w in worlds:
+            w.step(plain_steps)
+        ctx.sync()
+        plain_s = time.perf_counter() - t0
+        batch_rate = cells * steps / (batch_ms * 1e-3) / 1e6
+        plain_rate = cells * plain_steps / plain_s / 1e6
+        gbs = bytes_step * steps / (batch_ms * 1e-3) / 1e9
+        return {"worlds": len(descs), "cells": cells, "steps": steps,
+                "graph_steps": graph_steps, "ms_per_step": round(batch_ms / steps, 5),
+                "mcell_steps_per_s": round(batch_rate, 1),
+                "plain_mcell_steps_per_s": round(plain_rate, 1),
+                "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+                "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "bytes_per_step": int(bytes_step)}
+    finally:
+        for w in worlds:
+            w.close()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +119,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 256-world batched sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -101,6 +147,7 @@ def main() -> int:
         passes = w.pass_times(min(args.steps, 500))
         tiles = w.resolved()
         A = w.read()
+        batched = None if args.no_sweep else sweep_bench(ctx)
     sec = gpu_ms / 1e3
     value = cells * args.steps / sec / 1e6
     bpc = fft_bytes_per_cell(W, K)
@@ -137,6 +184,8 @@ def main() -> int:
                      "step_bytes_per_cell": round(bpc["step"], 3)},
         "state_mean": float(A.mean()),
     }
+    if batched is not None:
+        out["batched"] = batched
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, args.cpu_seconds)
     print(json.dumps(out))

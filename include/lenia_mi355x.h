@@ -83,10 +83,13 @@ typedef struct lenia_world_desc {
     uint32_t initMode; /* LENIA_INIT_* */
     uint32_t initX0, initY0, initW, initH; /* patch rectangle; initW or initH 0 => whole field */
     float initCoverage, initScale;         /* value-noise init */
-    /* FFT plan override per axis: radices packed 4 bits each, first stage in
-     * the lowest nibble, 0 = library planner.  Tile overrides: 0 = autotuned */
+    /* FFT launch overrides (0 = library choice).  Plans: radices packed 4
+     * bits each, first stage in the lowest nibble, 0xF encodes 16 and 0xE
+     * encodes 32.  Tiles: row pairs per row-pass workgroup, columns per
+     * column-pass workgroup, and the two workgroup sizes (64, 128 or 256). */
     uint32_t fftPlanW, fftPlanH;
     uint32_t rowPairsPerTile, colsPerTile;
+    uint32_t rowThreads, colThreads;
 } lenia_world_desc;
 
 typedef struct lenia_ctx lenia_ctx;

@@ -1,13 +1,14 @@
 // LDS-resident mixed-radix Stockham FFT for gfx950.
 //
-// A workgroup transforms `nt` independent complex sequences of length N held
-// back to back in LDS.  Each stage of radix R (2, 3, 4, 5 or 8) reads the R
-// inputs of each butterfly at stride N/R, twiddles them with e^{-2 pi i k r /
-// (Ns R)} from a table generated in FP64, applies the size-R DFT and writes
-// the outputs in autosort order, so the result lands in natural order with no
-// bit reversal.  This is the Stockham formulation the reference's radix-2
-// shader uses (Shaders/Common/FFT.metal:13-45, fftStageX/Y :65-121), widened to
-// mixed radix and kept inside one workgroup: one global read and one global
+// A workgroup of T threads transforms `nt` independent complex sequences of
+// length N held back to back in LDS.  Each stage of radix R (2, 3, 4, 5, 8, 16
+// or 32) reads the R inputs of each butterfly at stride N/R, twiddles them
+// with e^{-2 pi i k r / (Ns R)} from an FP64-generated table held in LDS,
+// applies the size-R DFT in registers and writes the outputs in autosort
+// order, so the result lands in natural order with no bit reversal.  This is
+// the Stockham formulation of the reference's radix-2 shader
+// (Shaders/Common/FFT.metal:13-45, fftStageX/Y :65-121), widened to mixed and
+// high radix and kept inside one workgroup: one global read and one global
 // write per transform instead of one full-field round trip per stage.
 //
 // All arithmetic is written with explicit fmaf and the file is compiled with
@@ -37,6 +38,22 @@ __device__ __forceinline__ float2 ctw(float2 a, float2 w) {
 template <bool INV>
 __device__ __forceinline__ float2 rot(float2 a) {
     return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// W32^e = e^{-2 pi i e / 32}, e < 32 (symmetric literals: exact 0 and +-1)
+__device__ __forceinline__ float2 w32(int e) {
+    constexpr float c[9] = {1.000000000e+00f, 9.807852804e-01f, 9.238795325e-01f,
+                            8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f,
+                            3.826834324e-01f, 1.950903220e-01f, 0.0f};
+    // cos(e pi/16) and sin(e pi/16) by quadrant; e is a compile-time constant
+    // at every call site, so this folds to literals
+    const int q = e >> 3, r = e & 7;
+    float co, si;
+    if (q == 0) { co = c[r]; si = c[8 - r]; }
+    else if (q == 1) { co = -c[8 - r]; si = c[r]; }
+    else if (q == 2) { co = -c[r]; si = -c[8 - r]; }
+    else { co = c[8 - r]; si = -c[r]; }
+    return make_float2(co, -si);
 }
 
 template <int R, bool INV>
@@ -89,6 +106,62 @@ struct Dft<8, INV> {
     }
 };
 
+// 16 = 4 x 4: x[4 n1 + n2], X[k1 + 4 k2] =
+//   sum_n2 W4^{n2 k2} W16^{n2 k1} sum_n1 x[4 n1 + n2] W4^{n1 k1}
+template <bool INV>
+struct Dft<16, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            float2 t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            Dft<4, INV>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+        }
+#pragma unroll
+        for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+            for (int k1 = 1; k1 < 4; ++k1) y[n2][k1] = ctw<INV>(y[n2][k1], w32(2 * n2 * k1));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            Dft<4, INV>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+        }
+    }
+};
+
+// 32 = 8 x 4: x[4 n1 + n2] (n1 < 8, n2 < 4), X[k1 + 8 k2] =
+//   sum_n2 W4^{n2 k2} W32^{n2 k1} sum_n1 x[4 n1 + n2] W8^{n1 k1}
+template <bool INV>
+struct Dft<32, INV> {
+    __device__ __forceinline__ static void run(float2 *v) {
+        float2 y[4][8];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            float2 t[8];
+#pragma unroll
+            for (int n1 = 0; n1 < 8; ++n1) t[n1] = v[4 * n1 + n2];
+            Dft<8, INV>::run(t);
+#pragma unroll
+            for (int k1 = 0; k1 < 8; ++k1) y[n2][k1] = t[k1];
+        }
+#pragma unroll
+        for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+            for (int k1 = 1; k1 < 8; ++k1) y[n2][k1] = ctw<INV>(y[n2][k1], w32(n2 * k1));
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+            float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            Dft<4, INV>::run(t);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[k1 + 8 * k2] = t[k2];
+        }
+    }
+};
+
 template <bool INV>
 struct Dft<3, INV> {
     __device__ __forceinline__ static void run(float2 *v) {
@@ -128,66 +201,102 @@ struct Dft<5, INV> {
     }
 };
 
-// One Stockham stage over nt transforms of length N in LDS.  Every thread
-// first loads all its butterflies, the group synchronises, then each thread
-// computes and stores, so the stage runs in place in one LDS buffer.
-// Precondition (checked on the host): ceil(nt*N/R / T) <= MAXV / R.
-template <int R, bool INV, int MAXV, int T>
-__device__ __forceinline__ void fft_stage(float2 *buf, int N, int Ns, int nt,
-                                          const float2 *__restrict__ tw, int tid) {
-    constexpr int MB = MAXV / R;
-    const int NR = N / R;
-    const int nb = nt * NR;
-    const int twStep = N / (Ns * R);
-    float2 v[MB][R];
-#pragma unroll
-    for (int i = 0; i < MB; ++i) {
-        int g = tid + i * T;
-        if (g < nb) {
-            int b = g / NR, j = g - b * NR;
-            const float2 *src = buf + b * N + j;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[i][r] = src[r * NR];
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < MB; ++i) {
-        int g = tid + i * T;
-        if (g < nb) {
-            int b = g / NR, j = g - b * NR;
-            int k = j % Ns;
-            if (k != 0) {
-#pragma unroll
-                for (int r = 1; r < R; ++r) v[i][r] = ctw<INV>(v[i][r], tw[k * r * twStep]);
-            }
-            Dft<R, INV>::run(v[i]);
-            float2 *dst = buf + b * N + (j - k) * R + k;
-#pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * Ns] = v[i][r];
-        }
-    }
-    __syncthreads();
+// floor(g / d) for g < 2^16, d < 2^16 with the host-computed magic
+// m = ceil(2^32 / d) (m = 0 encodes d = 1)
+__device__ __forceinline__ uint32_t fast_div(uint32_t g, uint32_t m) {
+    return m ? __umulhi(g, m) : g;
 }
 
-// Full transform; the caller synchronises after filling `buf`.  Unscaled in
-// both directions.
+// LDS swizzle of transform buffers: within every 32-element (256-byte) row of
+// the LDS banks the low index bits are XORed with the row number.  Stockham
+// stores land at stride R (up to 16 x 8 bytes); unswizzled, 32 of the 64 lanes
+// hit one bank.  With the swizzle every store and load of a stage is at most
+// 2-way (the minimum for 8-byte accesses).  A bijection within each row, so
+// buffers keep their size.
+__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 5) & 31); }
+
+// One Stockham stage over nt transforms in LDS (swizzled).  Every thread
+// first loads all its butterflies, the group synchronises, then each thread
+// twiddles, transforms and stores, so the stage runs in place in one LDS
+// buffer.  `tws` is this stage's twiddle table, [k][r-1] for k < Ns,
+// 0 < r < R.  Precondition (host): ceil(nt*N/R / T) * R <= MAXV.
+template <int R, bool INV, int MAXV, int T>
+__device__ __forceinline__ void fft_stage(float2 *buf, const FftStage &st, int N, int nt,
+                                          const float2 *tws, int tid) {
+    constexpr int MB = MAXV / R;
+    if constexpr (MB > 0) {
+        const int NR = st.NR, Ns = st.Ns;
+        const int nb = nt * NR;
+        float2 v[MB][R];
+        float2 w[MB][R - 1];
+        // data and twiddles are read in one batch (one LDS wait per stage);
+        // the twiddle tables are never written during a transform
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            int g = tid + i * T;
+            if (g < nb) {
+                int b = (int)fast_div((uint32_t)g, st.mNR);
+                int j = g - b * NR;
+                int k = j - Ns * (int)fast_div((uint32_t)j, st.mNs);
+                const int base = b * N + j;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = buf[sw(base + r * NR)];
+                const float2 *t = tws + k * (R - 1);
+#pragma unroll
+                for (int r = 0; r < R - 1; ++r) w[i][r] = t[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            int g = tid + i * T;
+            if (g < nb) {
+                int b = (int)fast_div((uint32_t)g, st.mNR);
+                int j = g - b * NR;
+                int k = j - Ns * (int)fast_div((uint32_t)j, st.mNs);
+                if (k != 0) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[i][r] = ctw<INV>(v[i][r], w[i][r - 1]);
+                }
+                Dft<R, INV>::run(v[i]);
+                const int base = b * N + (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[sw(base + r * Ns)] = v[i][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Full transform; the caller synchronises after filling `buf` and the LDS
+// stage-twiddle tables `tw` (plan.st[s].twOff indexes them).  Unscaled both
+// ways.
 template <bool INV, int MAXV, int T>
-__device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt,
-                                        const float2 *__restrict__ tw, int tid) {
-    int Ns = 1;
+__device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
+                                        int tid) {
     const int N = plan.n;
     for (int s = 0; s < plan.nStages; ++s) {
-        const int R = plan.radix[s];
-        switch (R) {
-            case 2: fft_stage<2, INV, MAXV, T>(buf, N, Ns, nt, tw, tid); break;
-            case 3: fft_stage<3, INV, MAXV, T>(buf, N, Ns, nt, tw, tid); break;
-            case 4: fft_stage<4, INV, MAXV, T>(buf, N, Ns, nt, tw, tid); break;
-            case 5: fft_stage<5, INV, MAXV, T>(buf, N, Ns, nt, tw, tid); break;
-            default: fft_stage<8, INV, MAXV, T>(buf, N, Ns, nt, tw, tid); break;
+        const FftStage &st = plan.st[s];
+        const float2 *tws = tw + st.twOff;
+        switch (st.radix) {
+            case 2: fft_stage<2, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 3: fft_stage<3, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 4: fft_stage<4, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 5: fft_stage<5, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 8: fft_stage<8, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 16: fft_stage<16, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            default: fft_stage<32, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
         }
-        Ns *= R;
     }
+}
+
+// XCD-aware tile order: consecutive workgroup ids land on different XCDs
+// (round robin over 8), so give each XCD a contiguous run of tiles; tiles
+// that share cache lines (neighbouring row pairs) then share one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int x = b & 7, j = b >> 3;
+    const int base = n >> 3, rem = n & 7;
+    return x * base + min(x, rem) + j;
 }
 
 }  // namespace lenia

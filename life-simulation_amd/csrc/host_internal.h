@@ -32,7 +32,12 @@ struct lenia_world {
     uint32_t frameNext = 0;   // frame index the next lenia_world_step starts at
     // FFT form
     lenia::FftPlan planW{}, planH{};
-    int P = 1, C = 1;
+    int P = 1, C = 1;       // row pairs / columns per tile
+    int rowT = 256, colT = 256;  // workgroup sizes
+    bool rowKpar = false, colKpar = false;  // K inverse transforms side by side
+    hipGraphExec_t graphExec = nullptr;     // captured step sequence
+    bool graphExtras = false;
+    uint32_t version = 0;  // bumped on every descriptor upload (batches re-read)
     float *F = nullptr, *G = nullptr, *S = nullptr, *twW = nullptr, *twH = nullptr;
     // direct form
     int half = 0;
@@ -51,6 +56,9 @@ struct lenia_world {
     size_t ldsRows = 0, ldsCols = 0;
 
     uint32_t parity() const { return (uint32_t)(hostSteps & 1u); }
+    bool extras() const {
+        return desc.noiseAmount > 0.0f || desc.muJitter > 0.0f || desc.injectAmount > 0.0f;
+    }
     float *current_state() const { return (isFft || parity() == 0) ? A0 : A1; }
 };
 
@@ -65,13 +73,24 @@ double kernel_raw(const lenia_kernel_desc &k, int dx, int dy);
 int kernel_half(const lenia_kernel_desc &k);
 bool kernel_window(const lenia_kernel_desc &k, int h, std::vector<double> &out);
 
-bool plan_default(int n, FftPlan &p);
+bool factor235(int n, int &a, int &b, int &c);
+bool plan_make(int n, const int *radix, int ns, FftPlan &p);
+bool plan_default(int n, int maxR, FftPlan &p);
 bool plan_unpack(int n, uint32_t packed, FftPlan &p);
 uint32_t plan_pack(const FftPlan &p);
 bool plan_fits(const FftPlan &p, int nt, int T, int maxv);
+size_t lds_rows(int W, int P, int K, bool kpar);
+size_t lds_cols(int H, int C, int K, bool kpar);
+bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
+bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
+                int &ntOut, int &TOut, bool &kparOut);
+hipError_t enqueue_step(lenia_world *w, hipEvent_t mid);
+int world_ensure_spectrum(lenia_world *w);
+int world_apply_init(lenia_world *w);
 
 int world_upload_desc(lenia_world *w);
 int world_launch_steps(lenia_world *w, uint32_t n);
+void world_drop_graph(lenia_world *w);
 
 uint32_t pcg_hash_host(uint32_t v);
 uint32_t module_seed(uint32_t sceneSeed);

@@ -20,31 +20,6 @@ constexpr int kDTX = 64;
 constexpr int kDM = 4;    // outputs per thread (vertical)
 constexpr int kDTY = 4 * kDM;
 
-template <class WorldT>
-__device__ __forceinline__ void direct_arrive(uint32_t *done, const WorldT *worlds, int nWorlds) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            for (int i = 0; i < nWorlds; ++i) {
-                StepClock *cp = worlds[i].clock;
-                StepClock c = *cp;
-                c.sub += 1u;
-                if (c.sub >= c.substeps) {
-                    c.sub = 0u;
-                    c.frame += 1u;
-                }
-                c.parity ^= 1u;
-                c.steps += 1u;
-                *cp = c;
-            }
-            __threadfence();
-            atomicExch(done, 0u);
-        }
-    }
-}
 
 __device__ __forceinline__ int wrap_i(int p, int n) {
     int r = p % n;
@@ -68,13 +43,13 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const int x0 = (tile.y % wd.tilesX) * kDTX;
         const int y0 = (tile.y / wd.tilesX) * kDTY;
         const int pitch = kDTX + 2 * h, rows = kDTY + 2 * h;
-        const StepClock ck = *wd.clock;
-        const float *src = ck.parity ? wd.A1 : wd.A0;
-        float *dst = ck.parity ? wd.A0 : wd.A1;
+        const StepClock ck = read_clock(wd.clock);
+        const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
+        LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
         for (int i = tid; i < rows * pitch; i += kDT) {
             int ly = i / pitch, lx = i - ly * pitch;
             int gy = wrap_i(y0 - h + ly, H), gx = wrap_i(x0 - h + lx, W);
-            tileS[i] = src[(size_t)gy * W + gx];
+            tileS[i] = src[gy * W + gx];
         }
         __syncthreads();
 
@@ -87,7 +62,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const int kBeg = MODE == 1 ? kSel : 0;
         const int kEnd = MODE == 1 ? kSel + 1 : (int)g.nKernels;
         for (int k = kBeg; k < kEnd; ++k) {
-            const float *__restrict__ wk = wd.win + (size_t)k * n * n;
+            const LENIA_GLOBAL float *wk = gp(wd.win) + k * n * n;
             float tot[kDM];
 #pragma unroll
             for (int j = 0; j < kDM; ++j) tot[j] = 0.0f;
@@ -114,7 +89,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
             for (int j = 0; j < kDM; ++j) {
                 int y = y0 + ty * kDM + j;
                 if (MODE == 1) {
-                    if (x < W && y < H) out[(size_t)y * W + x] = tot[j];
+                    if (x < W && y < H) gp(out)[y * W + x] = tot[j];
                 } else if (x < W && y < H) {
                     float muJ = cell_jitter(g, x, y, seed, ck.frame);
                     acc[j] += growth_term(g, k, tot[j], muJ);
@@ -127,13 +102,13 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
                 int y = y0 + ty * kDM + j;
                 if (x < W && y < H) {
                     float a = tileS[(ty * kDM + j + h) * pitch + tx + h];
-                    dst[(size_t)y * W + x] = growth_finish(g, a, acc[j], x, y, W, H, seed, ck.frame);
+                    dst[y * W + x] = growth_finish(g, a, acc[j], x, y, W, H, seed, ck.frame);
                 }
             }
         }
         __syncthreads();
     }
-    if (MODE == 0) direct_arrive(done, worlds, nWorlds);
+    if (MODE == 0) arrive_step(done, worlds, nWorlds);
 }
 
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,

@@ -16,8 +16,11 @@
 //
 // Spectra are stored column-major ([column][row]) so the column pass reads and
 // writes whole contiguous columns and the row pass moves two adjacent rows
-// (16 bytes) per spectrum column.  The inverse 1/(W H) is folded into the
-// kernel spectra, which are computed in FP64 from the kernel window.
+// (16 bytes) per spectrum column; tiles are mapped so that neighbouring row
+// pairs run on one XCD and share its L2.  Every global operand of a tile is
+// requested before the FFT that precedes its use, so loads overlap the LDS
+// work.  The inverse 1/(W H) is folded into the kernel spectra, which are
+// computed in FP64 from the kernel window.
 #include <hip/hip_runtime.h>
 
 #include "fft_device.h"
@@ -26,184 +29,395 @@
 
 namespace lenia {
 
-constexpr int kT = 256;    // threads per workgroup
-constexpr int kMaxV = 16;  // complex values per thread per transform batch
+constexpr int kMaxV = 16;  // complex values per thread per stage
 
-template <class WorldT>
-__device__ __forceinline__ void arrive_all(uint32_t *done, const WorldT *worlds, int nWorlds) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t prev = atomicAdd(done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            for (int i = 0; i < nWorlds; ++i) {
-                StepClock *cp = worlds[i].clock;
-                StepClock c = *cp;
-                c.sub += 1u;
-                if (c.sub >= c.substeps) {
-                    c.sub = 0u;
-                    c.frame += 1u;
-                }
-                c.parity ^= 1u;
-                c.steps += 1u;
-                *cp = c;
-            }
-            __threadfence();
-            atomicExch(done, 0u);
-        }
-    }
+__device__ __forceinline__ int tile_of(int ti, int nTiles) {
+    return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
 }
 
 // ---------------------------------------------------------------- row pass
-// mode: 0 = forward only (state -> F), 1 = step (G -> growth -> state -> F),
-//       2 = potential of kernel `kSel` only (G_kSel -> out), no state change
-template <int MODE>
-__global__ __launch_bounds__(kT) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
-                                                 const int2 *__restrict__ tiles, int nTiles,
-                                                 uint32_t *done, int nWorlds, int kSel,
-                                                 float *__restrict__ out) {
+// Twiddle-table copy global -> LDS, split in two so its loads can be in
+// flight together with the other loads of a tile
+template <int T>
+struct TwCopy {
+    static constexpr int B = 8;
+    float2 v[B];
+    __device__ __forceinline__ void issue(const LENIA_GLOBAL float2 *src, int n, int tid) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            int t = tid + i * T;
+            if (t < n) v[i] = gld2(src + t);
+        }
+    }
+    __device__ __forceinline__ void store(float2 *dst, const LENIA_GLOBAL float2 *src, int n,
+                                          int tid) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            int t = tid + i * T;
+            if (t < n) dst[t] = v[i];
+        }
+        for (int t = tid + B * T; t < n; t += T) dst[t] = gld2(src + t);
+    }
+};
+
+// Stored half-spectra of a tile's row pairs (kernels kBeg .. kBeg+kn-1) ->
+// full-length complex rows Z = Ga + i Gb, one LDS slot of nt rows per kernel,
+// with the Hermitian symmetry of each real row's spectrum for kx > W/2.
+// gather_issue puts the loads in flight; gather_store writes LDS.
+constexpr int kMG = kMaxV / 2 + 1;  // float4 loads per thread
+
+template <int T>
+__device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], const LENIA_GLOBAL float4 *G4,
+                                             size_t gStride, int kBeg, int kn, int H2, int nt,
+                                             int P, uint32_t mP, int nm, int tid) {
+    const int total = kn * nm;
+#pragma unroll
+    for (int i = 0; i < kMG; ++i) {
+        int m = tid + i * T;
+        if (m < total) {
+            int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
+            int mm = m - s * nm;
+            int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
+            if (p < nt) gv[i] = gld4(G4 + (size_t)(kBeg + s) * gStride + kx * H2 + p);
+        }
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[kMG], int kn, int W,
+                                             int nt, int P, uint32_t mP, int nm, int tid) {
+    const int total = kn * nm;
+#pragma unroll
+    for (int i = 0; i < kMG; ++i) {
+        int m = tid + i * T;
+        if (m < total) {
+            int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
+            int mm = m - s * nm;
+            int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
+            if (p < nt) {
+                float4 g4 = gv[i];
+                if (kx == 0 || kx == W / 2) {
+                    // real columns for a real field: drop the rounding
+                    // residue instead of leaking it across rows
+                    g4.y = 0.0f;
+                    g4.w = 0.0f;
+                }
+                const int base = (s * nt + p) * W;
+                Z[sw(base + kx)] = make_float2(g4.x - g4.w, g4.y + g4.z);
+                if (kx > 0 && kx < W / 2)
+                    Z[sw(base + W - kx)] = make_float2(g4.x + g4.w, g4.z - g4.y);
+            }
+        }
+    }
+}
+
+// forward FFT of the tile's (new) row pairs, split into the two real rows'
+// half-spectra Fa = (Z[k] + conj Z[W-k]) / 2, Fb = (Z[k] - conj Z[W-k]) / 2i
+template <int T>
+__device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, const float2 *tw,
+                                             int rp0, int nt, int tid) {
+    const int W = wd.W, H2 = wd.H / 2, P = wd.P;
+    const int nm = P * wd.Wh;
+    fft_lds<false, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+    LENIA_GLOBAL float4 *F4 = gp(reinterpret_cast<float4 *>(wd.F)) + rp0;
+    for (int m = tid; m < nm; m += T) {
+        int kx = (int)fast_div((uint32_t)m, wd.mP), p = m - kx * P;
+        if (p >= nt) continue;
+        float2 z1 = Z[sw(p * W + kx)];
+        float2 z2 = Z[sw(p * W + (kx == 0 ? 0 : W - kx))];
+        gst4(F4 + kx * H2 + p, make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
+                                           0.5f * (z1.y + z2.y), 0.5f * (z2.x - z1.x)));
+    }
+}
+
+// MODE 0: forward only (state -> F); 1: step (G -> growth -> state -> F);
+// 2: potential of kernel kSel only (G_kSel -> out), no state change.
+// KPAR: the K inverse transforms of a tile run side by side (one LDS slot per
+// kernel, spread over the workgroup's waves); otherwise they run one after
+// the other with the growth sums kept in LDS.
+// EXTRAS: the noise / muJitter / inject terms are compiled in (with all three
+// amounts zero both variants compute the same bits).
+// LDS: twiddles [W] | KPAR: Z [K][nt][W] ; serial: Z [nt][W] (sums in registers)
+template <int MODE, int T, bool EXTRAS, bool KPAR>
+__global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles,
+                                                uint32_t *done, int nWorlds, int kSel,
+                                                float *__restrict__ out) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
-        const int2 tile = tiles[ti];
+        const int2 tile = tiles[tile_of(ti, nTiles)];
         const FftWorldDev &wd = worlds[tile.x];
-        const int W = wd.W, H = wd.H, Wh = wd.Wh;
-        const int rp0 = tile.y * wd.P;
-        const int nt = min(wd.P, H / 2 - rp0);
-        float2 *Z = smem;
-        const float2 *tw = reinterpret_cast<const float2 *>(wd.twW);
-        float *A = wd.A;
+        const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
+        const int rp0 = tile.y * P;
+        const int nt = min(P, H2 - rp0);
+        const uint32_t mP = wd.mP;
+        float2 *tw = smem;
+        float2 *Z = smem + W;
+        LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
         const int nq = nt * W;
+        const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
+        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twW));
+        TwCopy<T> twc;
+        twc.issue(twg, W, tid);
 
         if (MODE == 0) {
-            for (int q = tid; q < nq; q += kT) {
-                int p = q / W, x = q - p * W;
-                int ya = 2 * (rp0 + p);
-                Z[q] = make_float2(A[(size_t)ya * W + x], A[(size_t)(ya + 1) * W + x]);
+            twc.store(tw, twg, W, tid);
+            for (int q = tid; q < nq; q += T) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ia = 2 * p * W + x;
+                Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
             }
             __syncthreads();
-        } else {
-            const GrowthParams &g = wd.g;
-            const StepClock ck = *wd.clock;
-            const uint32_t seed = clock_seed(ck, g.moduleSeed);
-            float acc0[kMaxV], acc1[kMaxV];
-#pragma unroll
-            for (int i = 0; i < kMaxV; ++i) acc0[i] = acc1[i] = 0.0f;
-            const int kBeg = MODE == 2 ? kSel : 0;
-            const int kEnd = MODE == 2 ? kSel + 1 : (int)g.nKernels;
-            for (int k = kBeg; k < kEnd; ++k) {
-                const float4 *G4 = reinterpret_cast<const float4 *>(wd.G) + (size_t)k * Wh * (H / 2);
-                for (int m = tid; m < nt * Wh; m += kT) {
-                    int kx = m / nt, p = m - kx * nt;
-                    float4 g4 = G4[(size_t)kx * (H / 2) + rp0 + p];
-                    if (kx == 0 || kx == W / 2) {
-                        // these columns are real for a real field: drop the
-                        // rounding residue instead of leaking it across rows
-                        g4.y = 0.0f;
-                        g4.w = 0.0f;
-                    }
-                    // Z = Ga + i Gb on the stored half, Hermitian mirror on the rest
-                    Z[p * W + kx] = make_float2(g4.x - g4.w, g4.y + g4.z);
-                    if (kx > 0 && kx < W / 2)
-                        Z[p * W + W - kx] = make_float2(g4.x + g4.w, g4.z - g4.y);
-                }
-                __syncthreads();
-                fft_lds<true, kMaxV, kT>(Z, wd.planW, nt, tw, tid);
-                if (MODE == 2) {
-                    for (int q = tid; q < nq; q += kT) {
-                        int p = q / W, x = q - p * W;
-                        int ya = 2 * (rp0 + p);
-                        float2 z = Z[q];
-                        out[(size_t)ya * W + x] = z.x;
-                        out[(size_t)(ya + 1) * W + x] = z.y;
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < kMaxV; ++i) {
-                        int q = tid + i * kT;
-                        if (q < nq) {
-                            int p = q / W, x = q - p * W;
-                            int ya = 2 * (rp0 + p);
-                            float2 z = Z[q];
-                            float ja = cell_jitter(g, x, ya, seed, ck.frame);
-                            float jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
-                            acc0[i] += growth_term(g, k, z.x, ja);
-                            acc1[i] += growth_term(g, k, z.y, jb);
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            if (MODE == 2) continue;
+            forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+            __syncthreads();
+            continue;
+        }
+        const GrowthParams &g = wd.g;
+        const StepClock ck = read_clock(wd.clock);
+        const uint32_t seed = clock_seed(ck, g.moduleSeed);
+        const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
+        const size_t gStride = (size_t)Wh * H2;
+        float ar0[kMaxV], ar1[kMaxV];
+        if (MODE == 1) {
 #pragma unroll
             for (int i = 0; i < kMaxV; ++i) {
-                int q = tid + i * kT;
+                int q = tid + i * T;
                 if (q < nq) {
-                    int p = q / W, x = q - p * W;
-                    int ya = 2 * (rp0 + p);
-                    size_t ia = (size_t)ya * W + x, ib = ia + W;
-                    float a2 = growth_finish(g, A[ia], acc0[i], x, ya, W, H, seed, ck.frame);
-                    float b2 = growth_finish(g, A[ib], acc1[i], x, ya + 1, W, H, seed, ck.frame);
-                    A[ia] = a2;
-                    A[ib] = b2;
-                    Z[q] = make_float2(a2, b2);
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    ar0[i] = Arow[ia];
+                    ar1[i] = Arow[ia + W];
                 }
             }
-            __syncthreads();
         }
-        // forward FFT of the (new) row pair; split into the two real rows'
-        // half-spectra: Fa = (Z[k] + conj Z[W-k]) / 2, Fb = (Z[k] - conj Z[W-k]) / 2i
-        fft_lds<false, kMaxV, kT>(Z, wd.planW, nt, tw, tid);
-        float4 *F4 = reinterpret_cast<float4 *>(wd.F);
-        for (int m = tid; m < nt * Wh; m += kT) {
-            int kx = m / nt, p = m - kx * nt;
-            float2 z1 = Z[p * W + kx];
-            float2 z2 = Z[p * W + (kx == 0 ? 0 : W - kx)];
-            F4[(size_t)kx * (H / 2) + rp0 + p] =
-                make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y), 0.5f * (z1.y + z2.y),
-                            0.5f * (z2.x - z1.x));
+        if (KPAR || MODE == 2) {
+            const int kBeg = MODE == 2 ? kSel : 0;
+            const int kn = MODE == 2 ? 1 : (int)g.nKernels;
+            float4 gv[kMG];
+            gather_issue<T>(gv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, tid);
+            twc.store(tw, twg, W, tid);
+            gather_store<T>(Z, gv, kn, W, nt, P, mP, nm, tid);
+            __syncthreads();
+            fft_lds<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
+            if (MODE == 2) {
+                for (int q = tid; q < nq; q += T) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    float2 z = Z[sw(q)];
+                    LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
+                    o[ia] = z.x;
+                    o[ia + W] = z.y;
+                }
+                __syncthreads();
+                continue;
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ya = 2 * (rp0 + p);
+                    float ja = 0.0f, jb = 0.0f;
+                    if (EXTRAS) {
+                        ja = cell_jitter(g, x, ya, seed, ck.frame);
+                        jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
+                    }
+                    float sa = 0.0f, sb = 0.0f;
+                    for (int k = 0; k < kn; ++k) {
+                        float2 z = Z[sw(k * nq + q)];
+                        sa += growth_term(g, k, z.x, ja);
+                        sb += growth_term(g, k, z.y, jb);
+                    }
+                    if (EXTRAS) {
+                        ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
+                        ar1[i] = growth_finish(g, ar1[i], sb, x, ya + 1, W, H, seed, ck.frame);
+                    } else {
+                        ar0[i] = growth_plain(g, ar0[i], sa);
+                        ar1[i] = growth_plain(g, ar1[i], sb);
+                    }
+                }
+            }
+            // stores after all LDS reads (no read waits behind a store)
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    Arow[ia] = ar0[i];
+                    Arow[ia + W] = ar1[i];
+                    Z[sw(q)] = make_float2(ar0[i], ar1[i]);
+                }
+            }
+        } else {
+            float sa[kMaxV], sb[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) sa[i] = sb[i] = 0.0f;
+            float4 gv[kMG];
+            gather_issue<T>(gv, G4, gStride, 0, 1, H2, nt, P, mP, nm, tid);
+            twc.store(tw, twg, W, tid);
+            for (int k = 0; k < (int)g.nKernels; ++k) {
+                gather_store<T>(Z, gv, 1, W, nt, P, mP, nm, tid);
+                // the next kernel's spectrum loads fly during this transform
+                if (k + 1 < (int)g.nKernels)
+                    gather_issue<T>(gv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, tid);
+                __syncthreads();
+                fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+#pragma unroll
+                for (int i = 0; i < kMaxV; ++i) {
+                    int q = tid + i * T;
+                    if (q < nq) {
+                        int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                        int ya = 2 * (rp0 + p);
+                        float2 z = Z[sw(q)];
+                        float ja = 0.0f, jb = 0.0f;
+                        if (EXTRAS) {
+                            ja = cell_jitter(g, x, ya, seed, ck.frame);
+                            jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
+                        }
+                        sa[i] += growth_term(g, k, z.x, ja);
+                        sb[i] += growth_term(g, k, z.y, jb);
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ya = 2 * (rp0 + p);
+                    float a2, b2;
+                    if (EXTRAS) {
+                        a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
+                        b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
+                    } else {
+                        a2 = growth_plain(g, ar0[i], sa[i]);
+                        b2 = growth_plain(g, ar1[i], sb[i]);
+                    }
+                    int ia = 2 * p * W + x;
+                    Arow[ia] = a2;
+                    Arow[ia + W] = b2;
+                    Z[sw(q)] = make_float2(a2, b2);
+                }
+            }
         }
         __syncthreads();
+        forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+        __syncthreads();
     }
-    if (MODE == 1) arrive_all(done, worlds, nWorlds);
+    if (MODE == 1) arrive_step(done, worlds, nWorlds);
 }
 
 // ---------------------------------------------------------------- column pass
-__global__ __launch_bounds__(kT) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
-                                                 const int2 *__restrict__ tiles, int nTiles) {
+// LDS: twiddles [H] | X [C][H] | KPAR: Y [K][C][H]
+template <int T, bool KPAR>
+__global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
-        const int2 tile = tiles[ti];
+        const int2 tile = tiles[tile_of(ti, nTiles)];
         const FftWorldDev &wd = worlds[tile.x];
         const int H = wd.H, Wh = wd.Wh;
         const int c0 = tile.y * wd.C;
         const int nt = min(wd.C, Wh - c0);
         const int ne = nt * H;
-        float2 *X = smem;
-        const float2 *tw = reinterpret_cast<const float2 *>(wd.twH);
-        const float2 *F = reinterpret_cast<const float2 *>(wd.F) + (size_t)c0 * H;
-        for (int e = tid; e < ne; e += kT) X[e] = F[e];
+        const int K = (int)wd.g.nKernels;
+        float2 *tw = smem;
+        float2 *X = smem + H;
+        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twH));
+        TwCopy<T> twc;
+        twc.issue(twg, H, tid);
+        const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + (size_t)c0 * H;
+        const LENIA_GLOBAL float *S0 = gp(wd.S) + (size_t)c0 * H;
+        const size_t sStride = (size_t)Wh * H;
+        LENIA_GLOBAL float2 *G0 = gp(reinterpret_cast<float2 *>(wd.G)) + (size_t)c0 * H;
+        {
+            float2 fv[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) fv[i] = gld2(F + e);
+            }
+            twc.store(tw, twg, H, tid);
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) X[sw(e)] = fv[i];
+            }
+        }
+        if (KPAR) {
+            float2 *Y = X + ((ne + 31) & ~31);  // own 32-aligned swizzle rows
+            const int nk = K * ne;
+            float sr[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) {
+                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                    sr[i] = S0[(size_t)k * sStride + c];
+                }
+            }
+            __syncthreads();
+            fft_lds<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            // all reads of the forward spectrum first, then the scaled copies
+            float2 xv[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) {
+                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                    xv[i] = X[sw(c)];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
+            }
+            __syncthreads();
+            fft_lds<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
+            for (int e = tid; e < nk; e += T) {
+                int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                gst2(G0 + (size_t)k * sStride + c, Y[sw(e)]);
+            }
+            __syncthreads();
+            continue;
+        }
+        float sr[kMaxV];
+#pragma unroll
+        for (int i = 0; i < kMaxV; ++i) {
+            int e = tid + i * T;
+            if (e < ne) sr[i] = S0[e];
+        }
         __syncthreads();
-        fft_lds<false, kMaxV, kT>(X, wd.planH, nt, tw, tid);
+        fft_lds<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
         float2 xs[kMaxV];
 #pragma unroll
         for (int i = 0; i < kMaxV; ++i) {
-            int e = tid + i * kT;
-            xs[i] = e < ne ? X[e] : make_float2(0.f, 0.f);
+            int e = tid + i * T;
+            xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
         }
-        for (int k = 0; k < (int)wd.g.nKernels; ++k) {
-            const float *S = wd.S + ((size_t)k * Wh + c0) * H;
+        for (int k = 0; k < K; ++k) {
 #pragma unroll
             for (int i = 0; i < kMaxV; ++i) {
-                int e = tid + i * kT;
-                if (e < ne) X[e] = cscale(xs[i], S[e]);
+                int e = tid + i * T;
+                if (e < ne) X[sw(e)] = cscale(xs[i], sr[i]);
+            }
+            if (k + 1 < K) {
+                const LENIA_GLOBAL float *Sn = S0 + (size_t)(k + 1) * sStride;
+#pragma unroll
+                for (int i = 0; i < kMaxV; ++i) {
+                    int e = tid + i * T;
+                    if (e < ne) sr[i] = Sn[e];
+                }
             }
             __syncthreads();
-            fft_lds<true, kMaxV, kT>(X, wd.planH, nt, tw, tid);
-            float2 *G = reinterpret_cast<float2 *>(wd.G) + ((size_t)k * Wh + c0) * H;
-            for (int e = tid; e < ne; e += kT) G[e] = X[e];
+            fft_lds<true, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            LENIA_GLOBAL float2 *G = G0 + (size_t)k * sStride;
+            for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
             __syncthreads();
         }
     }
@@ -216,6 +430,22 @@ __global__ void k_twiddles(float2 *tw, int N) {
     if (t >= N) return;
     double a = 2.0 * (double)t / (double)N;
     tw[t] = make_float2((float)cospi(a), (float)(-sinpi(a)));
+}
+
+// per-stage twiddle tables of a plan: stage s holds e^{-2 pi i k r / (Ns R)}
+// at [twOff + k (R-1) + r - 1], k < Ns, 0 < r < R, copied from the length-N
+// FP64-evaluated table
+__global__ void k_stage_twiddles(FftPlan plan, const float2 *__restrict__ raw, float2 *out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int s = 0; s < plan.nStages; ++s) {
+        const FftStage &st = plan.st[s];
+        const int size = st.Ns * (st.radix - 1);
+        if (t >= st.twOff && t < st.twOff + size) {
+            int u = t - st.twOff;
+            int k = u / (st.radix - 1), r = u - k * (st.radix - 1) + 1;
+            out[t] = raw[k * r * st.twStep];
+        }
+    }
 }
 
 // first half of the separable kernel spectrum, FP64:
@@ -255,30 +485,80 @@ __global__ void k_spec_cols(const double *__restrict__ Cw, int h, int H, int Wh,
 }
 
 // ---------------------------------------------------------------- launchers
-hipError_t launch_fft_rows(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t *done, int nWorlds, int kSel,
-                           float *out, hipStream_t s) {
+template <int T, bool EX, bool KP>
+static void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                        int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                        hipStream_t s) {
     if (mode == 0)
-        hipLaunchKernelGGL(k_fft_rows<0>, dim3(grid), dim3(kT), lds, s, worlds, tiles, nTiles,
-                           done, nWorlds, kSel, out);
+        hipLaunchKernelGGL((k_fft_rows<0, T, false, false>), dim3(grid), dim3(T), lds, s, worlds,
+                           tiles, nTiles, done, nWorlds, kSel, out);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_fft_rows<1>, dim3(grid), dim3(kT), lds, s, worlds, tiles, nTiles,
-                           done, nWorlds, kSel, out);
+        hipLaunchKernelGGL((k_fft_rows<1, T, EX, KP>), dim3(grid), dim3(T), lds, s, worlds, tiles,
+                           nTiles, done, nWorlds, kSel, out);
     else
-        hipLaunchKernelGGL(k_fft_rows<2>, dim3(grid), dim3(kT), lds, s, worlds, tiles, nTiles,
-                           done, nWorlds, kSel, out);
+        hipLaunchKernelGGL((k_fft_rows<2, T, false, true>), dim3(grid), dim3(T), lds, s, worlds,
+                           tiles, nTiles, done, nWorlds, kSel, out);
+}
+
+template <int T>
+static void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
+                          const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
+                          int nWorlds, int kSel, float *out, hipStream_t s) {
+    if (extras) {
+        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+    } else {
+        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+    }
+}
+
+hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
+                           const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
+                           size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                           hipStream_t s) {
+    switch (threads) {
+        case 64: rows_dispatch<64>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 128: rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 256: rows_dispatch<256>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 512: rows_dispatch<512>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_fft_cols(const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
-                           size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL(k_fft_cols, dim3(grid), dim3(kT), lds, s, worlds, tiles, nTiles);
+template <int T>
+static void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                          int grid, size_t lds, hipStream_t s) {
+    if (kpar)
+        hipLaunchKernelGGL((k_fft_cols<T, true>), dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+    else
+        hipLaunchKernelGGL((k_fft_cols<T, false>), dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+}
+
+hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
+                           int nTiles, int grid, size_t lds, hipStream_t s) {
+    switch (threads) {
+        case 64: cols_dispatch<64>(kpar, worlds, tiles, nTiles, grid, lds, s); break;
+        case 128: cols_dispatch<128>(kpar, worlds, tiles, nTiles, grid, lds, s); break;
+        case 256: cols_dispatch<256>(kpar, worlds, tiles, nTiles, grid, lds, s); break;
+        case 512: cols_dispatch<512>(kpar, worlds, tiles, nTiles, grid, lds, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s) {
     hipLaunchKernelGGL(k_twiddles, dim3((N + 255) / 256), dim3(256), 0, s,
                        reinterpret_cast<float2 *>(tw), N);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_twiddles(const FftPlan &plan, const float *raw, float *out,
+                                  hipStream_t s) {
+    const int n = plan.n;  // the stage tables hold n - 1 entries
+    hipLaunchKernelGGL(k_stage_twiddles, dim3((n + 255) / 256), dim3(256), 0, s, plan,
+                       reinterpret_cast<const float2 *>(raw), reinterpret_cast<float2 *>(out));
     return hipGetLastError();
 }
 
@@ -294,7 +574,6 @@ hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double
     return hipGetLastError();
 }
 
-int fft_threads() { return kT; }
 int fft_max_values() { return kMaxV; }
 
 }  // namespace lenia

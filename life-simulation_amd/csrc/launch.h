@@ -10,15 +10,16 @@ namespace lenia {
 
 // FFT path (k_fft.hip).  mode 0: forward row FFT of the state; 1: full step
 // row pass; 2: potential of kernel kSel into `out`.
-hipError_t launch_fft_rows(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t *done, int nWorlds, int kSel,
-                           float *out, hipStream_t s);
-hipError_t launch_fft_cols(const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
-                           size_t lds, hipStream_t s);
+hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
+                           const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
+                           size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                           hipStream_t s);
+hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
+                           int nTiles, int grid, size_t lds, hipStream_t s);
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s);
+hipError_t launch_stage_twiddles(const FftPlan &plan, const float *raw, float *out, hipStream_t s);
 hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double *Cw, float *S,
                                   hipStream_t s);
-int fft_threads();
 int fft_max_values();
 
 // direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.

@@ -1,13 +1,306 @@
-// Batched stepping of many worlds (placeholder until the batch table lands).
+// Batched stepping of many independent worlds (the PresetLab sweep, which the
+// reference runs one SceneRunner per variant, one after the other:
+// Apps/LifePresetLab/main.cpp:718-806).
+//
+// A batch keeps a device table of world descriptors and one tile list per
+// pass spanning every world; a step is one column-pass launch and one
+// row-pass launch over all tiles (FFT form) or one launch (direct form), and
+// step sequences are replayed from a captured hipGraph.  Each world keeps its
+// own buffers, FFT plans and step clock, so a world gives the same bits
+// stepped alone or in any batch: the plan of a length never depends on the
+// launch shape, and tile shapes only change which threads run a butterfly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 #include "host_internal.h"
+#include "launch.h"
 
 using namespace lenia;
 
-extern "C" int lenia_batch_create(lenia_ctx *, lenia_world *const *, uint32_t, lenia_batch **out) {
-    if (out) *out = nullptr;
-    return fail(LENIA_ERR_UNSUPPORTED, "batches are not built yet");
+struct lenia_batch {
+    lenia_ctx *ctx = nullptr;
+    std::vector<lenia_world *> worlds;
+    bool isFft = false;
+    bool extras = false;
+    int rowT = 256, colT = 256;
+    bool rowKpar = true, colKpar = true;
+    void *devWorlds = nullptr;
+    int2 *rowTiles = nullptr, *colTiles = nullptr;
+    int nRowTiles = 0, nColTiles = 0;
+    size_t ldsRows = 0, ldsCols = 0;
+    uint32_t *done = nullptr;
+    hipGraphExec_t graph = nullptr;
+    uint32_t graphSteps = 0;
+    std::vector<uint32_t> versions;  // world descriptor versions the table holds
+};
+
+static void batch_release_tables(lenia_batch *b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    dev_free(b->ctx, b->devWorlds);
+    dev_free(b->ctx, b->rowTiles);
+    dev_free(b->ctx, b->colTiles);
+    b->devWorlds = nullptr;
+    b->rowTiles = b->colTiles = nullptr;
+    b->ldsRows = b->ldsCols = 0;
 }
-extern "C" int lenia_batch_destroy(lenia_batch *) { return LENIA_OK; }
-extern "C" int lenia_batch_step(lenia_batch *, uint32_t, uint32_t) {
-    return fail(LENIA_ERR_UNSUPPORTED, "batches are not built yet");
+
+static void batch_release(lenia_batch *b) {
+    batch_release_tables(b);
+    dev_free(b->ctx, b->done);
+    b->done = nullptr;
+}
+
+static int batch_setup_fft(lenia_batch *b);
+static int batch_setup_direct(lenia_batch *b);
+
+// (re)build the world table and tiles from the worlds' current descriptors
+static int batch_setup(lenia_batch *b) {
+    batch_release_tables(b);
+    b->versions.clear();
+    for (lenia_world *w : b->worlds) b->versions.push_back(w->version);
+    return b->isFft ? batch_setup_fft(b) : batch_setup_direct(b);
+}
+
+// transforms per tile for one world at workgroup size T: about 12 values per
+// thread, a power of two, and a shape the plan, registers and LDS allow
+static int batch_nt(const FftPlan &p, bool rows, int count, int K, int T, bool kpar) {
+    int target = std::max(1, (12 * T) / ((kpar ? K : 1) * p.n));
+    int nt = 1;
+    while (nt * 2 <= target && nt * 2 <= count && nt * 2 <= 64) nt *= 2;
+    while (nt > 1 && !shape_ok(p, rows, nt, K, T, kpar)) nt /= 2;
+    return shape_ok(p, rows, nt, K, T, kpar) ? nt : 0;
+}
+
+static int batch_setup_fft(lenia_batch *b) {
+    lenia_ctx *ctx = b->ctx;
+    const int n = (int)b->worlds.size();
+    // one workgroup size per pass for the whole batch; side-by-side kernels
+    // when every world allows it
+    const int Ts[4] = {256, 512, 128, 64};
+    bool found = false;
+    for (int kp = 1; kp >= 0 && !found; --kp)
+        for (int T : Ts) {
+            bool ok = true;
+            for (lenia_world *w : b->worlds) {
+                const int K = (int)w->desc.nKernels;
+                if (!batch_nt(w->planW, true, w->desc.height / 2, K, T, kp != 0)) ok = false;
+            }
+            if (ok) {
+                b->rowT = T;
+                b->rowKpar = kp != 0;
+                found = true;
+                break;
+            }
+        }
+    if (!found) return fail(LENIA_ERR_UNSUPPORTED, "batch: no common row-pass shape");
+    found = false;
+    for (int kp = 1; kp >= 0 && !found; --kp)
+        for (int T : Ts) {
+            bool ok = true;
+            for (lenia_world *w : b->worlds) {
+                const int K = (int)w->desc.nKernels;
+                if (!batch_nt(w->planH, false, w->desc.width / 2 + 1, K, T, kp != 0)) ok = false;
+            }
+            if (ok) {
+                b->colT = T;
+                b->colKpar = kp != 0;
+                found = true;
+                break;
+            }
+        }
+    if (!found) return fail(LENIA_ERR_UNSUPPORTED, "batch: no common column-pass shape");
+
+    std::vector<FftWorldDev> table(n);
+    std::vector<int2> rt, ct;
+    for (int i = 0; i < n; ++i) {
+        lenia_world *w = b->worlds[i];
+        FftWorldDev f = w->fdev;  // same buffers, plans, clock; batch tile shape
+        const int K = (int)w->desc.nKernels;
+        f.P = batch_nt(w->planW, true, f.H / 2, K, b->rowT, b->rowKpar);
+        f.C = batch_nt(w->planH, false, f.Wh, K, b->colT, b->colKpar);
+        f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
+        f.colTiles = (f.Wh + f.C - 1) / f.C;
+        f.mP = f.P <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + f.P - 1) / f.P);
+        f.rowThreads = b->rowT;
+        f.colThreads = b->colT;
+        table[i] = f;
+        for (int t = 0; t < f.rowTiles; ++t) rt.push_back(make_int2(i, t));
+        for (int t = 0; t < f.colTiles; ++t) ct.push_back(make_int2(i, t));
+        b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar), lds_rows(f.W, f.P, 1, true)});
+        b->ldsCols = std::max(b->ldsCols, lds_cols(f.H, f.C, K, b->colKpar));
+    }
+    b->nRowTiles = (int)rt.size();
+    b->nColTiles = (int)ct.size();
+    int rc = dev_alloc(ctx, &b->devWorlds, sizeof(FftWorldDev) * n);
+    if (!rc) rc = dev_alloc(ctx, (void **)&b->rowTiles, sizeof(int2) * rt.size());
+    if (!rc) rc = dev_alloc(ctx, (void **)&b->colTiles, sizeof(int2) * ct.size());
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(FftWorldDev) * n,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->rowTiles, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->colTiles, ct.data(), sizeof(int2) * ct.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? LENIA_OK : hip_fail(e, "batch upload");
+}
+
+static int batch_setup_direct(lenia_batch *b) {
+    lenia_ctx *ctx = b->ctx;
+    const int n = (int)b->worlds.size();
+    std::vector<DirectWorldDev> table(n);
+    std::vector<int2> rt;
+    for (int i = 0; i < n; ++i) {
+        lenia_world *w = b->worlds[i];
+        table[i] = w->ddev;
+        for (int t = 0; t < w->ddev.tilesX * w->ddev.tilesY; ++t) rt.push_back(make_int2(i, t));
+        b->ldsRows = std::max(b->ldsRows, w->ldsRows);
+    }
+    b->nRowTiles = (int)rt.size();
+    int rc = dev_alloc(ctx, &b->devWorlds, sizeof(DirectWorldDev) * n);
+    if (!rc) rc = dev_alloc(ctx, (void **)&b->rowTiles, sizeof(int2) * rt.size());
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(DirectWorldDev) * n,
+                                  hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->rowTiles, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? LENIA_OK : hip_fail(e, "batch upload");
+}
+
+extern "C" int lenia_batch_create(lenia_ctx *ctx, lenia_world *const *worlds, uint32_t n,
+                                  lenia_batch **out) {
+    if (!ctx || !worlds || !out || n == 0)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_create: bad argument");
+    *out = nullptr;
+    (void)hipSetDevice(ctx->device);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!worlds[i]) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_create: world %u is NULL", i);
+        if (worlds[i]->ctx != ctx)
+            return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_create: world %u has another context", i);
+        if (worlds[i]->desc.convMode != worlds[0]->desc.convMode)
+            return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_create: all worlds need one convMode");
+        for (uint32_t j = 0; j < i; ++j)
+            if (worlds[j] == worlds[i])
+                return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_create: world %u listed twice", i);
+    }
+    if (worlds[0]->isMfma)
+        return fail(LENIA_ERR_UNSUPPORTED, "lenia_batch_create: MFMA forms are stepped alone");
+    lenia_batch *b = new lenia_batch();
+    b->ctx = ctx;
+    b->worlds.assign(worlds, worlds + n);
+    b->isFft = worlds[0]->isFft;
+    for (lenia_world *w : b->worlds) b->extras = b->extras || w->extras();
+    int rc = dev_alloc(ctx, (void **)&b->done, sizeof(uint32_t) * 4);
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(b->done, 0, sizeof(uint32_t) * 4, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "batch counter");
+    }
+    if (!rc) rc = batch_setup(b);
+    if (rc) {
+        batch_release(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_destroy(lenia_batch *b) {
+    if (!b) return LENIA_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_release(b);
+    delete b;
+    return LENIA_OK;
+}
+
+static hipError_t batch_enqueue_step(lenia_batch *b) {
+    lenia_ctx *ctx = b->ctx;
+    const int n = (int)b->worlds.size();
+    if (b->isFft) {
+        hipError_t e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds,
+                                       b->colTiles, b->nColTiles, b->nColTiles, b->ldsCols,
+                                       ctx->stream);
+        if (e != hipSuccess) return e;
+        return launch_fft_rows(1, b->rowT, b->extras, b->rowKpar,
+                               (const FftWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
+                               b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
+    }
+    return launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
+                         b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
+}
+
+extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: NULL batch");
+    if (graphSteps && nSteps % graphSteps)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: nSteps %u is not a multiple of %u",
+                    nSteps, graphSteps);
+    lenia_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    bool stale = false;
+    for (size_t i = 0; i < b->worlds.size(); ++i) stale = stale || b->versions[i] != b->worlds[i]->version;
+    if (stale) {
+        // a world's parameters changed since the table was built
+        b->extras = false;
+        for (lenia_world *w : b->worlds) b->extras = b->extras || w->extras();
+        int rc = batch_setup(b);
+        if (rc) return rc;
+    }
+    for (lenia_world *w : b->worlds) {
+        int rc = LENIA_OK;
+        if (w->needsInit) {
+            rc = world_apply_init(w);
+            hipError_t e = rc ? hipSuccess
+                              : launch_set_clock(w->clock, w->frameNext, 0, 1, w->parity(), ctx->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "set clock");
+        }
+        if (!rc) rc = world_ensure_spectrum(w);
+        if (rc) return rc;
+    }
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    if (graphSteps) {
+        if (!b->graph || b->graphSteps != graphSteps) {
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return hip_fail(e, "batch graph capture");
+            hipError_t el = hipSuccess;
+            for (uint32_t i = 0; i < graphSteps && el == hipSuccess; ++i) el = batch_enqueue_step(b);
+            e = hipStreamEndCapture(ctx->stream, &g);
+            if (el != hipSuccess) e = el;
+            if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            if (g) (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph = nullptr;
+                return hip_fail(e, "batch graph instantiate");
+            }
+            b->graphSteps = graphSteps;
+        }
+        for (uint32_t r = 0; r < nSteps / graphSteps; ++r) {
+            hipError_t e = hipGraphLaunch(b->graph, ctx->stream);
+            if (e != hipSuccess) return hip_fail(e, "batch graph launch");
+        }
+    } else {
+        for (uint32_t i = 0; i < nSteps; ++i) {
+            hipError_t e = batch_enqueue_step(b);
+            if (e != hipSuccess) return hip_fail(e, "batch step launch");
+        }
+    }
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    ctx->timed = true;
+    for (lenia_world *w : b->worlds) {
+        w->hostSteps += nSteps;
+        w->frameNext += nSteps;
+    }
+    return LENIA_OK;
 }

@@ -9,6 +9,45 @@
 
 namespace lenia {
 
+// Pointers loaded from a world descriptor are generic (flat) to the compiler;
+// flat accesses share a wait counter with LDS and make every LDS wait also
+// wait for HBM.  Cast them to the global address space.
+#define LENIA_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ LENIA_GLOBAL T *gp(T *p) {
+    return (LENIA_GLOBAL T *)p;
+}
+// vector loads / stores through global pointers (HIP's float2/float4 cannot
+// be copy-constructed from another address space; clang vectors can)
+typedef float lv2 __attribute__((ext_vector_type(2)));
+typedef float lv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld4(const LENIA_GLOBAL float4 *p) {
+    lv4 v = *(const LENIA_GLOBAL lv4 *)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst4(LENIA_GLOBAL float4 *p, float4 a) {
+    *(LENIA_GLOBAL lv4 *)p = lv4{a.x, a.y, a.z, a.w};
+}
+__device__ __forceinline__ float2 gld2(const LENIA_GLOBAL float2 *p) {
+    lv2 v = *(const LENIA_GLOBAL lv2 *)p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void gst2(LENIA_GLOBAL float2 *p, float2 a) {
+    *(LENIA_GLOBAL lv2 *)p = lv2{a.x, a.y};
+}
+// frame / sub of a world's step clock
+__device__ __forceinline__ StepClock read_clock(const StepClock *c) {
+    const LENIA_GLOBAL StepClock *g = gp(c);
+    StepClock v;
+    v.frame = g->frame;
+    v.sub = g->sub;
+    v.substeps = g->substeps;
+    v.parity = g->parity;
+    v.steps = g->steps;
+    v.pad[0] = v.pad[1] = v.pad[2] = 0u;
+    return v;
+}
+
 // pcg_hash (Shaders/Common/Random.metal:5-9)
 __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
     uint32_t state = v * 747796405u + 2891336453u;
@@ -43,14 +82,18 @@ __device__ __forceinline__ float cell_jitter(const GrowthParams &g, uint32_t x, 
     return 0.0f;
 }
 
-__device__ __forceinline__ float bell_of(float u, float mu, float sigma) {
-    float t = (u - mu) / sigma;
-    return expf(-0.5f * t * t);
+// bell = exp(-((u-mu)/sigma)^2 / 2) evaluated as exp2(-t*t) with
+// t = (u-mu) * sqrt(log2(e)/2) / sigma (scale precomputed in FP64): one
+// multiply and the native exp2 instead of an IEEE divide and expf.  Within
+// 3 ulp of the FP64 bell.
+__device__ __forceinline__ float bell_of(float u, float mu, float scale) {
+    float t = (u - mu) * scale;
+    return __builtin_amdgcn_exp2f(-(t * t));
 }
 
 // one kernel's contribution, added in kernel order
 __device__ __forceinline__ float growth_term(const GrowthParams &g, int k, float u, float muJ) {
-    float b = bell_of(u, g.mu[k] + muJ, g.sigma[k]);
+    float b = bell_of(u, g.mu[k] + muJ, g.bellScale[k]);
     if (g.growthMode == 1u) return g.weight[k] * b;
     return g.weight[k] * (2.0f * b - 1.0f);
 }
@@ -91,38 +134,57 @@ __device__ __forceinline__ float growth_finish(const GrowthParams &g, float A, f
     return fminf(fmaxf(A2, 0.0f), 1.0f);
 }
 
+// growth_finish without the noise / inject terms (identical bits when their
+// amounts are zero)
+__device__ __forceinline__ float growth_plain(const GrowthParams &g, float A, float acc) {
+    float A2;
+    if (g.growthMode == 1u) {
+        float target = (g.sumWeights > 1e-6f) ? (acc / g.sumWeights) : A;
+        A2 = A + g.dt * (target - A);
+    } else {
+        A2 = A + g.dt * acc;
+    }
+    return fminf(fmaxf(A2, 0.0f), 1.0f);
+}
+
 // Step seed of the step a kernel is running: pcg(seed_ ^ (frame*197 + sub))
 // (Modules/FieldModules/Lenia.cpp:564)
 __device__ __forceinline__ uint32_t clock_seed(const StepClock &c, uint32_t moduleSeed) {
     return pcg_hash(moduleSeed ^ (c.frame * 197u + c.sub));
 }
 
-// Advance every clock once all blocks of the launch are done.  Called by
-// every block at the end of the last kernel of a step; the block that
-// finishes last moves the clocks on and re-arms the counter.  The next
-// kernel in stream order sees the new values.
-__device__ __forceinline__ void clock_arrive(uint32_t *done, StepClock *clocks, int nClocks) {
+// Advance the step clock of every world of a launch once all its blocks are
+// done.  Called by every block at the end of the last kernel of a step: the
+// block that arrives last (counter) moves the clocks on with the 64 lanes of
+// its first wave and re-arms the counter.  The next kernel in stream order
+// sees the new values.
+template <class WorldT>
+__device__ __forceinline__ void arrive_step(uint32_t *done, const WorldT *worlds, int nWorlds) {
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x >= 64) return;
+    const int lane = threadIdx.x;
+    int last = 0;
+    if (lane == 0) {
         __threadfence();
-        uint32_t prev = atomicAdd(done, 1u);
-        if (prev == gridDim.x - 1) {
-            __threadfence();
-            for (int i = 0; i < nClocks; ++i) {
-                StepClock c = clocks[i];
-                c.sub += 1u;
-                if (c.sub >= c.substeps) {
-                    c.sub = 0u;
-                    c.frame += 1u;
-                }
-                c.parity ^= 1u;
-                c.steps += 1u;
-                clocks[i] = c;
-            }
-            __threadfence();
-            atomicExch(done, 0u);
-        }
+        last = atomicAdd(done, 1u) == gridDim.x - 1 ? 1 : 0;
     }
+    last = __shfl(last, 0, 64);
+    if (!last) return;
+    __threadfence();
+    for (int i = lane; i < nWorlds; i += 64) {
+        LENIA_GLOBAL StepClock *cp = gp(worlds[i].clock);
+        uint32_t sub = cp->sub + 1u, frame = cp->frame;
+        if (sub >= cp->substeps) {
+            sub = 0u;
+            frame += 1u;
+        }
+        cp->sub = sub;
+        cp->frame = frame;
+        cp->parity ^= 1u;
+        cp->steps += 1u;
+    }
+    __threadfence();
+    if (lane == 0) atomicExch(done, 0u);
 }
 
 }  // namespace lenia

@@ -193,7 +193,7 @@ bool lenia::kernel_window(const lenia_kernel_desc &k, int h, std::vector<double>
 }
 
 // ------------------------------------------------------------------ FFT plans
-static bool factor235(int n, int &a, int &b, int &c) {
+bool lenia::factor235(int n, int &a, int &b, int &c) {
     a = b = c = 0;
     while (n % 2 == 0) { n /= 2; ++a; }
     while (n % 3 == 0) { n /= 3; ++b; }
@@ -201,62 +201,96 @@ static bool factor235(int n, int &a, int &b, int &c) {
     return n == 1;
 }
 
-bool lenia::plan_default(int n, FftPlan &p) {
-    int a, b, c;
-    if (n < 2 || !factor235(n, a, b, c)) return false;
-    std::vector<int> r;
-    int a8 = a / 3, rem = a % 3;
-    if (rem == 1 && a8 > 0) {
-        a8 -= 1;
-        for (int i = 0; i < a8; ++i) r.push_back(8);
-        r.push_back(4);
-        r.push_back(4);
-    } else {
-        for (int i = 0; i < a8; ++i) r.push_back(8);
-        if (rem == 2) r.push_back(4);
-        if (rem == 1) r.push_back(2);
+static uint32_t div_magic(int d) {
+    if (d <= 1) return 0u;
+    return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
+}
+
+// fills the per-stage constants of a radix sequence; false if the product is
+// not n or a division magic is not exact on the index range it serves
+bool lenia::plan_make(int n, const int *radix, int ns, FftPlan &p) {
+    if (ns < 1 || ns > kMaxStages) return false;
+    std::memset(&p, 0, sizeof(p));
+    long prod = 1;
+    int Ns = 1, twOff = 0;
+    for (int s = 0; s < ns; ++s) {
+        int R = radix[s];
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8 && R != 16 && R != 32) return false;
+        prod *= R;
+        if (n % (Ns * R) != 0) return false;
+        FftStage &st = p.st[s];
+        st.radix = R;
+        st.NR = n / R;
+        st.Ns = Ns;
+        st.twStep = n / (Ns * R);
+        st.twOff = twOff;
+        twOff += Ns * (R - 1);
+        st.mNR = div_magic(st.NR);
+        st.mNs = div_magic(Ns);
+        // the kernels divide indices below 2^16 (nt * N <= 4096 in practice)
+        for (uint32_t g = 0; g < 65536u; g += 1u) {
+            uint32_t q1 = st.mNR ? (uint32_t)(((uint64_t)g * st.mNR) >> 32) : g;
+            uint32_t q2 = st.mNs ? (uint32_t)(((uint64_t)g * st.mNs) >> 32) : g;
+            if (q1 != g / (uint32_t)st.NR || q2 != g / (uint32_t)Ns) return false;
+        }
+        Ns *= R;
     }
-    for (int i = 0; i < b; ++i) r.push_back(3);
-    for (int i = 0; i < c; ++i) r.push_back(5);
-    if ((int)r.size() > kMaxStages) return false;
+    if (prod != n) return false;
     p.n = n;
-    p.nStages = (int)r.size();
-    for (int i = 0; i < kMaxStages; ++i) p.radix[i] = i < p.nStages ? r[i] : 0;
+    p.nStages = ns;
     return true;
 }
 
-bool lenia::plan_unpack(int n, uint32_t packed, FftPlan &p) {
-    int prod = 1, s = 0;
-    for (; s < kMaxStages; ++s) {
-        int r = (packed >> (4 * s)) & 0xF;
-        if (r == 0) break;
-        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return false;
-        p.radix[s] = r;
-        prod *= r;
+// radix sequence for n: the power-of-two part in the fewest stages of radix
+// <= maxR (balanced, e.g. 1024 = 16*16*4, 512 = 8*8*8), then 3s and 5s
+bool lenia::plan_default(int n, int maxR, FftPlan &p) {
+    int a, b, c;
+    if (n < 2 || !factor235(n, a, b, c)) return false;
+    int lmax = 1;
+    while ((1 << (lmax + 1)) <= maxR && lmax < 5) ++lmax;
+    std::vector<int> r;
+    if (a > 0) {
+        int stages = (a + lmax - 1) / lmax;
+        int base = a / stages, extra = a % stages;
+        for (int s = 0; s < stages; ++s) r.push_back(1 << (base + (s < extra ? 1 : 0)));
     }
-    if (prod != n || s == 0) return false;
-    p.n = n;
-    p.nStages = s;
-    for (int i = s; i < kMaxStages; ++i) p.radix[i] = 0;
-    return true;
+    for (int i = 0; i < b; ++i) r.push_back(3);
+    for (int i = 0; i < c; ++i) r.push_back(5);
+    return plan_make(n, r.data(), (int)r.size(), p);
+}
+
+bool lenia::plan_unpack(int n, uint32_t packed, FftPlan &p) {
+    int r[kMaxStages], s = 0;
+    for (; s < kMaxStages; ++s) {
+        int v = (packed >> (4 * s)) & 0xF;
+        if (v == 0) break;
+        r[s] = v == 0xF ? 16 : (v == 0xE ? 32 : v);  // nibbles F/E encode 16/32
+    }
+    return plan_make(n, r, s, p);
 }
 
 uint32_t lenia::plan_pack(const FftPlan &p) {
     uint32_t v = 0;
-    for (int s = 0; s < p.nStages; ++s) v |= (uint32_t)p.radix[s] << (4 * s);
+    for (int s = 0; s < p.nStages; ++s) {
+        int R = p.st[s].radix;
+        uint32_t code = R == 16 ? 0xF : (R == 32 ? 0xE : (uint32_t)R);
+        v |= code << (4 * s);
+    }
     return v;
 }
 
 // can one workgroup of T threads run nt transforms of this plan with at most
 // maxv values per thread per stage?
 bool lenia::plan_fits(const FftPlan &p, int nt, int T, int maxv) {
+    if ((long)nt * p.n > (long)T * maxv) return false;
     for (int s = 0; s < p.nStages; ++s) {
-        int R = p.radix[s];
+        int R = p.st[s].radix;
+        if (R > maxv) return false;
         long nb = (long)nt * p.n / R;
         long perThread = (nb + T - 1) / T;
-        if (perThread * R > maxv || perThread > maxv / R) return false;
+        if (perThread * R > maxv) return false;
     }
-    return (long)nt * p.n <= (long)T * maxv;
+    return true;
 }
 
 // ------------------------------------------------------------------ descriptor checks
@@ -313,6 +347,7 @@ static GrowthParams growth_of(const lenia_world_desc *d, uint32_t moduleSeed) {
         bool on = k < (int)d->nKernels;
         g.mu[k] = on ? d->kernels[k].mu : 0.15f;
         g.sigma[k] = on ? d->kernels[k].sigma : 0.017f;
+        g.bellScale[k] = (float)(std::sqrt(0.5 / std::log(2.0)) / (double)g.sigma[k]);
         g.weight[k] = on ? d->kernels[k].weight : 0.0f;
         if (on) sw += g.weight[k];  // same accumulation order as leniaGrowthMulti
     }
@@ -382,42 +417,96 @@ static int build_kernels_fft(lenia_world *w) {
     return rc;
 }
 
-static void choose_tiles_fft(lenia_world *w, int numCUs) {
-    const lenia_world_desc &d = w->desc;
-    const int W = d.width, H = d.height, Wh = W / 2 + 1;
-    const int T = fft_threads(), mv = fft_max_values();
-    // largest row-pair count that fits, then shrink while the grid is
-    // smaller than the chip (more tiles beat fatter tiles until every CU
-    // has work)
-    int P = d.rowPairsPerTile;
-    if (P == 0) {
-        P = 1;
-        while (plan_fits(w->planW, P * 2, T, mv) && P * 2 <= H / 2) P *= 2;
-        while (P > 1 && (H / 2 + P - 1) / P < numCUs) P /= 2;
-    }
-    int C = d.colsPerTile;
-    if (C == 0) {
-        C = 1;
-        while (plan_fits(w->planH, C * 2, T, mv) && C * 2 <= Wh) C *= 2;
-        while (C > 1 && (Wh + C - 1) / C < numCUs) C /= 2;
-    }
-    w->P = P;
-    w->C = C;
+// Launch shape of one FFT pass: transforms per tile (nt), threads per
+// workgroup (T), whether the K inverse transforms of a tile run side by side
+// (kpar, K transforms per tile slot), and the radix plan.  Explicit values
+// from the descriptor win.  Otherwise: the smallest tile (nt = 1) with the
+// smallest workgroup that holds <= 16 values per thread, side by side when it
+// fits in 64 KB of LDS.
+static constexpr size_t kLdsMax = 65536;
+
+size_t lenia::lds_rows(int W, int P, int K, bool kpar) {
+    return (size_t)(W + (kpar ? K : 1) * P * W) * sizeof(float2);
+}
+size_t lenia::lds_cols(int H, int C, int K, bool kpar) {
+    return (size_t)(H + (kpar ? 1 + K : 1) * C * H + (kpar ? 32 : 0)) * sizeof(float2);
 }
 
-static int validate_tiles_fft(lenia_world *w) {
-    const int T = fft_threads(), mv = fft_max_values();
-    if (w->P < 1 || !plan_fits(w->planW, w->P, T, mv))
-        return fail(LENIA_ERR_INVALID_ARG, "row tile of %d row pairs does not fit a workgroup for W=%d",
-                    w->P, w->desc.width);
-    if (w->C < 1 || !plan_fits(w->planH, w->C, T, mv))
-        return fail(LENIA_ERR_INVALID_ARG, "column tile of %d columns does not fit a workgroup for H=%d",
-                    w->C, w->desc.height);
+// does (nt transforms per tile, T threads, side-by-side kernels or not) fit
+// this plan, the per-thread register budget and 64 KB of LDS?
+bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar) {
+    const int mv = fft_max_values();
+    const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
+    if (lds > kLdsMax) return false;
+    return plan_fits(p, kpar ? nt * K : nt, T, mv) && plan_fits(p, nt, T, mv);
+}
+
+// The radix plan of a length depends only on the length (or the explicit
+// override), never on the launch shape, so every shape -- and every batch a
+// world joins -- computes the same bits.  Largest radices first; smaller
+// ones only when no shape can run the plan.
+static bool choose_plan(int N, int K, uint32_t packed, FftPlan &p, bool rows) {
+    auto any_shape = [&](const FftPlan &q) {
+        for (int T = 64; T <= 512; T *= 2)
+            for (int kp = 0; kp < 2; ++kp)
+                if (shape_ok(q, rows, 1, K, T, kp != 0)) return true;
+        return false;
+    };
+    if (packed) return plan_unpack(N, packed, p) && any_shape(p);
+    for (int maxR = 16; maxR >= 2; maxR /= 2)
+        if (plan_default(N, maxR, p) && any_shape(p)) return true;
+    return false;
+}
+
+// Default launch shape: transforms side by side when they fit, one transform
+// (row pair / column) per tile, and the smallest workgroup holding at most
+// 12 values per thread.  Explicit descriptor values restrict the choice.
+bool lenia::pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt,
+                       uint32_t wantT, int &ntOut, int &TOut, bool &kparOut) {
+    for (int kp = 1; kp >= 0; --kp) {
+        for (int nt = 1; nt <= 64; nt *= 2) {
+            if (wantNt ? nt != (int)wantNt : nt > 1) continue;
+            if (nt > count) break;
+            int best = 0;
+            for (int T = 64; T <= 512; T *= 2) {
+                if (wantT && T != (int)wantT) continue;
+                if (!shape_ok(p, rows, nt, K, T, kp != 0)) continue;
+                best = T;
+                if ((kp ? nt * K : nt) * p.n <= 12 * T) break;
+            }
+            if (best) {
+                ntOut = nt;
+                TOut = best;
+                kparOut = kp != 0;
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+static int choose_tiles_fft(lenia_world *w, int numCUs) {
+    (void)numCUs;
+    const lenia_world_desc &d = w->desc;
+    const int W = d.width, H = d.height, Wh = W / 2 + 1, K = (int)d.nKernels;
+    if (!choose_plan(W, K, d.fftPlanW, w->planW, true) ||
+        !pick_shape(w->planW, true, H / 2, K, d.rowPairsPerTile, d.rowThreads, w->P, w->rowT,
+                    w->rowKpar))
+        return fail(LENIA_ERR_UNSUPPORTED,
+                    "no row-pass shape fits width %d (plan 0x%x, %u row pairs, %u threads)", W,
+                    d.fftPlanW, d.rowPairsPerTile, d.rowThreads);
+    if (!choose_plan(H, K, d.fftPlanH, w->planH, false) ||
+        !pick_shape(w->planH, false, Wh, K, d.colsPerTile, d.colThreads, w->C, w->colT,
+                    w->colKpar))
+        return fail(LENIA_ERR_UNSUPPORTED,
+                    "no column-pass shape fits height %d (plan 0x%x, %u columns, %u threads)", H,
+                    d.fftPlanH, d.colsPerTile, d.colThreads);
     return LENIA_OK;
 }
 
 int lenia::world_upload_desc(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
+    w->version += 1;
     if (w->isFft) {
         FftWorldDev f;
         std::memset(&f, 0, sizeof(f));
@@ -430,6 +519,11 @@ int lenia::world_upload_desc(lenia_world *w) {
         f.colTiles = (f.Wh + f.C - 1) / f.C;
         f.planW = w->planW;
         f.planH = w->planH;
+        f.K = (int)w->desc.nKernels;
+        f.mP = div_magic(f.P);
+        f.mW = div_magic(f.W);
+        f.rowThreads = w->rowT;
+        f.colThreads = w->colT;
         f.g = w->g;
         f.twW = w->twW;
         f.twH = w->twH;
@@ -450,8 +544,8 @@ int lenia::world_upload_desc(lenia_world *w) {
                                hipMemcpyHostToDevice, ctx->stream), "upload tiles");
         w->nRowTiles = f.rowTiles;
         w->nColTiles = f.colTiles;
-        w->ldsRows = (size_t)f.P * f.W * sizeof(float2);
-        w->ldsCols = (size_t)f.C * f.H * sizeof(float2);
+        w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
+        w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
     } else {
         DirectWorldDev dd;
         std::memset(&dd, 0, sizeof(dd));
@@ -483,6 +577,7 @@ int lenia::world_upload_desc(lenia_world *w) {
 
 static void world_release(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
+    world_drop_graph(w);
     void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF,
                     w->clock, w->done, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
     for (void *p : ptrs) dev_free(ctx, p);
@@ -505,21 +600,17 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
             delete w;
             return fail(LENIA_ERR_UNSUPPORTED, "FFT form needs even width and height (got %dx%d)", W, H);
         }
-        bool okW = desc->fftPlanW ? plan_unpack(W, desc->fftPlanW, w->planW) : plan_default(W, w->planW);
-        bool okH = desc->fftPlanH ? plan_unpack(H, desc->fftPlanH, w->planH) : plan_default(H, w->planH);
-        if (!okW || !okH) {
+        int a_, b_, c_;
+        if (!factor235(W, a_, b_, c_) || !factor235(H, a_, b_, c_)) {
             delete w;
             return fail(LENIA_ERR_UNSUPPORTED,
-                        "FFT form needs sides of the form 2^a 3^b 5^c with a valid radix plan "
-                        "(got %dx%d)", W, H);
+                        "FFT form needs sides of the form 2^a 3^b 5^c (got %dx%d)", W, H);
         }
-        if (!plan_fits(w->planW, 1, fft_threads(), fft_max_values()) ||
-            !plan_fits(w->planH, 1, fft_threads(), fft_max_values())) {
+        if (W > 4096 || H > 4096) {
             delete w;
-            return fail(LENIA_ERR_UNSUPPORTED, "FFT length above 4096 (got %dx%d)", W, H);
+            return fail(LENIA_ERR_UNSUPPORTED, "FFT form supports sides up to 4096 (got %dx%d)", W, H);
         }
-        choose_tiles_fft(w, ctx->numCUs);
-        rc = validate_tiles_fft(w);
+        rc = choose_tiles_fft(w, ctx->numCUs);
         if (rc) {
             delete w;
             return rc;
@@ -546,7 +637,8 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         if (!rc) rc = dev_alloc(ctx, (void **)&w->S, K * Wh * H * sizeof(float));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twW, (size_t)W * sizeof(float2));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twH, (size_t)H * sizeof(float2));
-        int nr = (H / 2 + w->P - 1) / w->P, nc = (Wh + w->C - 1) / w->C;
+        // sized for one transform per tile, the most tiles any shape uses
+        int nr = H / 2, nc = Wh;
         if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nr * sizeof(int2));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->colTiles, nc * sizeof(int2));
     } else {
@@ -565,8 +657,16 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
     if (e == hipSuccess && w->A1) e = launch_fill(w->A1, cells, 0.0f, ctx->stream);
     if (e == hipSuccess) e = launch_set_clock(w->clock, 0, 0, 1, 0, ctx->stream);
     if (e == hipSuccess && w->isFft) {
-        e = launch_twiddles(w->twW, W, ctx->stream);
-        if (e == hipSuccess) e = launch_twiddles(w->twH, H, ctx->stream);
+        // FP64-evaluated length-N tables, then the plans' per-stage tables
+        float *raw = nullptr;
+        int rc2 = dev_alloc(ctx, (void **)&raw, (size_t)std::max(W, H) * sizeof(float2));
+        e = rc2 ? hipErrorOutOfMemory : launch_twiddles(raw, W, ctx->stream);
+        if (e == hipSuccess) e = launch_stage_twiddles(w->planW, raw, w->twW, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess) e = launch_twiddles(raw, H, ctx->stream);
+        if (e == hipSuccess) e = launch_stage_twiddles(w->planH, raw, w->twH, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, raw);
         if (e == hipSuccess) e = launch_fill(w->G, K * Wh * H * 2, 0.0f, ctx->stream);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -609,7 +709,7 @@ extern "C" int lenia_world_reset(lenia_world *w, uint32_t sceneSeed) {
 }
 
 // init pass (the needsInit_ branch of LeniaModule::encode, Lenia.cpp:468-495)
-static int world_apply_init(lenia_world *w) {
+int lenia::world_apply_init(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
     const lenia_world_desc &d = w->desc;
     const int W = d.width, H = d.height;
@@ -633,13 +733,63 @@ static int world_apply_init(lenia_world *w) {
 }
 
 // F must hold the spectrum of the current state before a row+column step
-static int world_ensure_spectrum(lenia_world *w) {
+int lenia::world_ensure_spectrum(lenia_world *w) {
     if (!w->isFft || w->specValid) return LENIA_OK;
     lenia_ctx *ctx = w->ctx;
-    hipError_t e = launch_fft_rows(0, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+    hipError_t e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                                    w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "forward row pass");
     w->specValid = true;
+    return LENIA_OK;
+}
+
+// the kernels of one step on the context stream; with `mid` set, an event is
+// recorded between the column and row passes (per-pass timing)
+hipError_t lenia::enqueue_step(lenia_world *w, hipEvent_t mid) {
+    lenia_ctx *ctx = w->ctx;
+    hipError_t e;
+    if (w->isFft) {
+        e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles,
+                            w->nColTiles, w->nColTiles, w->ldsCols, ctx->stream);
+        if (e == hipSuccess && mid) e = hipEventRecord(mid, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
+                                (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                                w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+    } else {
+        e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                          w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+    }
+    return e;
+}
+
+void lenia::world_drop_graph(lenia_world *w) {
+    if (w->graphExec) (void)hipGraphExecDestroy(w->graphExec);
+    w->graphExec = nullptr;
+}
+
+// Steps are replayed from a hipGraph of kGraphSteps steps, captured once per
+// world: kernels read their parameters and step clock from device memory, so
+// one graph serves every later replay.  Leftover steps launch directly.
+static constexpr uint32_t kGraphSteps = 32;
+
+static int world_build_graph(lenia_world *w) {
+    lenia_ctx *ctx = w->ctx;
+    world_drop_graph(w);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+    hipError_t el = hipSuccess;
+    for (uint32_t i = 0; i < kGraphSteps && el == hipSuccess; ++i) el = enqueue_step(w, nullptr);
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (el != hipSuccess) e = el;
+    if (e == hipSuccess) e = hipGraphInstantiate(&w->graphExec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        w->graphExec = nullptr;
+        return hip_fail(e, "graph instantiate");
+    }
+    w->graphExtras = w->extras();
     return LENIA_OK;
 }
 
@@ -647,18 +797,20 @@ int lenia::world_launch_steps(lenia_world *w, uint32_t n) {
     lenia_ctx *ctx = w->ctx;
     int rc = world_ensure_spectrum(w);
     if (rc) return rc;
-    for (uint32_t i = 0; i < n; ++i) {
-        hipError_t e;
-        if (w->isFft) {
-            e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles,
-                                w->nColTiles, w->ldsCols, ctx->stream);
-            if (e == hipSuccess)
-                e = launch_fft_rows(1, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                    w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
-        } else {
-            e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                              w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+    if (n >= kGraphSteps) {
+        if (!w->graphExec || w->graphExtras != w->extras()) {
+            rc = world_build_graph(w);
+            if (rc) return rc;
         }
+        for (uint32_t r = 0; r < n / kGraphSteps; ++r) {
+            hipError_t e = hipGraphLaunch(w->graphExec, ctx->stream);
+            if (e != hipSuccess) return hip_fail(e, "graph launch");
+            w->hostSteps += kGraphSteps;
+        }
+        n %= kGraphSteps;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        hipError_t e = enqueue_step(w, nullptr);
         if (e != hipSuccess) return hip_fail(e, "step launch");
         w->hostSteps += 1;
     }
@@ -730,12 +882,16 @@ extern "C" int lenia_world_set_params(lenia_world *w, const lenia_world_desc *de
     }
     uint32_t keepPW = w->desc.fftPlanW, keepPH = w->desc.fftPlanH;
     uint32_t keepP = w->desc.rowPairsPerTile, keepC = w->desc.colsPerTile;
+    uint32_t keepRT = w->desc.rowThreads, keepCT = w->desc.colThreads;
     w->desc = *desc;
     w->desc.fftPlanW = keepPW;
     w->desc.fftPlanH = keepPH;
     w->desc.rowPairsPerTile = keepP;
     w->desc.colsPerTile = keepC;
+    w->desc.rowThreads = keepRT;
+    w->desc.colThreads = keepCT;
     w->g = growth_of(desc, w->moduleSeed);
+    world_drop_graph(w);
     if (shapes) {
         rc = w->isFft ? build_kernels_fft(w) : build_kernels_direct(w);
         if (rc) return rc;
@@ -788,10 +944,10 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
     if (w->isFft) {
         int rc = world_ensure_spectrum(w);
         if (rc) return rc;
-        e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles, w->nColTiles,
+        e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles, w->nColTiles,
                             w->ldsCols, ctx->stream);
         if (e == hipSuccess)
-            e = launch_fft_rows(2, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+            e = launch_fft_rows(2, w->rowT, false, true, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                                 w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut,
                                 ctx->stream);
     } else {
@@ -813,6 +969,8 @@ extern "C" int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out) {
         out->fftPlanH = plan_pack(w->planH);
         out->rowPairsPerTile = w->P;
         out->colsPerTile = w->C;
+        out->rowThreads = w->rowT;
+        out->colThreads = w->colT;
     }
     return LENIA_OK;
 }
@@ -832,6 +990,36 @@ uint32_t lenia::module_seed(uint32_t sceneSeed) {
 // LifeCore/Metal/GPUTimer.mm:28-94, averaged the way LifeBench reports them,
 // Apps/LifeBench/main.cpp:112-133).  Events are recorded on the context stream
 // between passes, so each figure is one kernel's duration.
+// times `body` (n repetitions captured in one graph, so the GPU runs them back
+// to back with no host gaps) with events around the graph launch
+template <class F>
+static hipError_t time_graph(lenia_ctx *ctx, uint32_t n, F body, double *ms) {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t eb = hipSuccess;
+    for (uint32_t i = 0; i < n && eb == hipSuccess; ++i) eb = body();
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (eb != hipSuccess) e = eb;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+    float f = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    *ms = f;
+    return e;
+}
+
+// Per-pass GPU time (GPUTimer per-pass timestamps, LifeCore/Metal/GPUTimer.mm:
+// 28-94, averaged as LifeBench reports them, Apps/LifeBench/main.cpp:112-133).
+// FFT form: the column pass alone is a pure function of the spectrum F, so
+// n repetitions of it are timed by themselves; then n full steps are timed
+// and the row pass is the difference.  Direct forms: one pass per step.
 extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs,
                                       uint32_t maxPasses, uint32_t *nPasses) {
     if (!w || !outMs || !nPasses || nSteps == 0)
@@ -846,39 +1034,25 @@ extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *o
     }
     int rc = world_ensure_spectrum(w);
     if (rc) return rc;
-    std::vector<hipEvent_t> ev((size_t)nSteps * np + 1);
-    for (auto &e : ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
-    hipError_t e = hipEventRecord(ev[0], ctx->stream);
-    size_t ei = 1;
-    for (uint32_t i = 0; i < nSteps && e == hipSuccess; ++i) {
-        if (w->isFft) {
-            e = launch_fft_cols((const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles,
-                                w->nColTiles, w->ldsCols, ctx->stream);
-            if (e == hipSuccess) e = hipEventRecord(ev[ei++], ctx->stream);
-            if (e == hipSuccess)
-                e = launch_fft_rows(1, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                    w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
-        } else {
-            e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                              w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
-        }
-        if (e == hipSuccess) e = hipEventRecord(ev[ei++], ctx->stream);
-        if (e == hipSuccess) w->hostSteps += 1;
+    double colsMs = 0.0, stepMs = 0.0;
+    hipError_t e = hipSuccess;
+    if (w->isFft) {
+        e = time_graph(ctx, nSteps, [&]() {
+            return launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc,
+                                   w->colTiles, w->nColTiles, w->nColTiles, w->ldsCols,
+                                   ctx->stream);
+        }, &colsMs);
     }
-    if (e == hipSuccess) e = hipEventSynchronize(ev[ei - 1]);
-    for (uint32_t p = 0; p < np; ++p) outMs[p] = 0.0;
-    if (e == hipSuccess) {
-        for (uint32_t i = 0; i < nSteps; ++i)
-            for (uint32_t p = 0; p < np; ++p) {
-                float ms = 0.f;
-                hipError_t e2 = hipEventElapsedTime(&ms, ev[(size_t)i * np + p], ev[(size_t)i * np + p + 1]);
-                if (e2 != hipSuccess) e = e2;
-                outMs[p] += ms / nSteps;
-            }
-    }
-    for (auto &x : ev) (void)hipEventDestroy(x);
-    w->frameNext += nSteps;
+    if (e == hipSuccess) e = time_graph(ctx, nSteps, [&]() { return enqueue_step(w, nullptr); }, &stepMs);
     if (e != hipSuccess) return hip_fail(e, "pass timing");
+    w->hostSteps += nSteps;
+    w->frameNext += nSteps;
+    if (w->isFft) {
+        outMs[0] = colsMs / nSteps;
+        outMs[1] = (stepMs - colsMs) / nSteps;
+    } else {
+        outMs[0] = stepMs / nSteps;
+    }
     *nPasses = np;
     return LENIA_OK;
 }

@@ -17,6 +17,7 @@ struct GrowthParams {
     float mu[kMaxKernels];
     float sigma[kMaxKernels];
     float weight[kMaxKernels];
+    float bellScale[kMaxKernels];  // sqrt(log2(e)/2) / sigma: bell = exp2(-(t*t))
     float sumWeights;
     uint32_t nKernels;
     uint32_t growthMode;
@@ -39,11 +40,27 @@ struct StepClock {
     uint32_t pad[3];
 };
 
-// FFT plan of one length: radices in stage order
+// one Stockham stage: radix R, butterflies per transform NR = N/R, span Ns
+// (product of the earlier radices), twiddle stride N/(Ns R) into the length-N
+// table, offset of this stage's [Ns][R-1] twiddle table, and division magics
+// ceil(2^32/d) for d = NR and d = Ns (0 when d = 1)
+struct FftStage {
+    int32_t radix, NR, Ns, twStep, twOff;
+    uint32_t mNR, mNs;
+    int32_t pad;
+};
+
+// FFT plan of one length: stages in order
 struct FftPlan {
     int32_t n;
     int32_t nStages;
-    int32_t radix[kMaxStages];
+    FftStage st[kMaxStages];
+};
+
+// launch shape of one pass: threads per workgroup and values per thread
+struct PassShape {
+    int32_t threads;  // 64, 128 or 256
+    int32_t maxv;     // 16 or 32
 };
 
 // device descriptor of one FFT-path world
@@ -52,7 +69,10 @@ struct FftWorldDev {
     int32_t P;         // row pairs per row-pass tile
     int32_t C;         // spectrum columns per column-pass tile
     int32_t rowTiles, colTiles;
-    int32_t pad0;
+    int32_t K;
+    uint32_t mP;                    // division magic for P
+    int32_t rowThreads, colThreads; // workgroup sizes of the two passes
+    uint32_t mW;                    // division magic for W
     FftPlan planW, planH;
     GrowthParams g;
     const float *twW;  // interleaved complex twiddles e^{-2 pi i t / W}, t < W

@@ -122,7 +122,8 @@ class World:
         d = _abi.WorldDesc()
         check(_abi.lib().lenia_world_get_desc(self._h, C.byref(d)))
         return {"fftPlanW": d.fftPlanW, "fftPlanH": d.fftPlanH,
-                "rowPairsPerTile": d.rowPairsPerTile, "colsPerTile": d.colsPerTile}
+                "rowPairsPerTile": d.rowPairsPerTile, "colsPerTile": d.colsPerTile,
+                "rowThreads": d.rowThreads, "colThreads": d.colThreads}
 
     def close(self):
         if self._h:

@@ -77,6 +77,8 @@ class WorldDesc(C.Structure):
         ("fftPlanH", C.c_uint32),
         ("rowPairsPerTile", C.c_uint32),
         ("colsPerTile", C.c_uint32),
+        ("rowThreads", C.c_uint32),
+        ("colThreads", C.c_uint32),
     ]
 
 
@@ -181,4 +183,6 @@ def world_desc(w: dict) -> WorldDesc:
     d.fftPlanH = int(w.get("fftPlanH", 0))
     d.rowPairsPerTile = int(w.get("rowPairsPerTile", 0))
     d.colsPerTile = int(w.get("colsPerTile", 0))
+    d.rowThreads = int(w.get("rowThreads", 0))
+    d.colThreads = int(w.get("colThreads", 0))
     return d

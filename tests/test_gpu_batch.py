@@ -1,0 +1,155 @@
+"""Batched stepping: every world of a batch gives the same bytes as the same
+world stepped alone, for the FFT and direct forms, with and without graph
+replay and with the per-frame noise / jitter / inject terms."""
+import numpy as np
+import pytest
+
+from lenia_cases import DIRECT, FFT, INIT_NONE, exp_shell, rand_state, ring, sweep_worlds
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import lenia
+
+    c = lenia.Context(0)
+    yield c
+    c.close()
+
+
+def run_alone(ctx, desc, seed, steps, init=None):
+    import lenia
+
+    with lenia.World(ctx, desc) as w:
+        w.reset(seed)
+        if init is not None:
+            w.write(init)
+        w.frame(0, 1)
+        w.step(steps)
+        return w.read()
+
+
+def run_batched(ctx, descs, seeds, steps, graph_steps, inits=None):
+    import lenia
+
+    worlds = [lenia.World(ctx, d) for d in descs]
+    try:
+        for i, (w, s) in enumerate(zip(worlds, seeds)):
+            w.reset(s)
+            if inits is not None:
+                w.write(inits[i])
+            w.frame(0, 1)
+        with lenia.Batch(ctx, worlds) as b:
+            b.step(steps, graph_steps)
+        return [w.read() for w in worlds]
+    finally:
+        for w in worlds:
+            w.close()
+
+
+@pytest.mark.parametrize("graph_steps", [0, 8])
+def test_fft_batch_bit_exact_vs_alone(ctx, graph_steps):
+    descs = sweep_worlds(24, seed=7)
+    seeds = [1000 + i for i in range(len(descs))]
+    got = run_batched(ctx, descs, seeds, 24, graph_steps)
+    for i, d in enumerate(descs):
+        want = run_alone(ctx, d, seeds[i], 24)
+        assert got[i].tobytes() == want.tobytes(), (i, d["width"], d["height"])
+        print(f"world {i} {d['width']}x{d['height']} K={len(d['kernels'])}: bit-exact, "
+              f"mass {float(want.sum()):.3f}")
+
+
+def test_direct_batch_bit_exact_vs_alone(ctx):
+    descs = []
+    for i, (W, H, R) in enumerate([(64, 64, 8), (96, 80, 10), (130, 70, 13), (200, 128, 20)]):
+        descs.append(dict(width=W, height=H, convMode=DIRECT, growthMode=0, dt=0.1, radius=R,
+                          kernels=[ring(R, [1, 0.5][: 1 + i % 2], 0.15 + 0.02 * i, 0.02, 1.0)],
+                          initMode=0, initRect=(W // 4, H // 4, W // 2, H // 2)))
+    seeds = [5, 6, 7, 8]
+    got = run_batched(ctx, descs, seeds, 17, 0)
+    for i, d in enumerate(descs):
+        assert got[i].tobytes() == run_alone(ctx, d, seeds[i], 17).tobytes(), i
+
+
+def test_batch_with_extras_bit_exact(ctx):
+    base = dict(width=96, height=64, convMode=FFT, growthMode=1, dt=0.35, radius=8,
+                noiseAmount=0.4, muJitter=0.5, injectAmount=0.3, injectCount=3,
+                kernels=[ring(8, [1], 0.15, 0.017, 1.0), ring(6, [1, 0.6], 0.3, 0.035, 0.6)],
+                initMode=0, initRect=(0, 0, 0, 0))
+    plain = dict(base, noiseAmount=0.0, muJitter=0.0, injectAmount=0.0)
+    descs = [base, plain, dict(base, width=128)]
+    seeds = [11, 12, 13]
+    got = run_batched(ctx, descs, seeds, 16, 8)
+    for i, d in enumerate(descs):
+        assert got[i].tobytes() == run_alone(ctx, d, seeds[i], 16).tobytes(), i
+
+
+def test_batch_sees_parameter_changes(ctx):
+    import lenia
+
+    d = sweep_worlds(4, seed=3)
+    worlds = [lenia.World(ctx, x) for x in d]
+    try:
+        for i, w in enumerate(worlds):
+            w.reset(i)
+            w.frame(0, 1)
+        with lenia.Batch(ctx, worlds) as b:
+            b.step(4)
+            d2 = dict(d[0], dt=d[0]["dt"] / 2)
+            worlds[0].set_params(d2)
+            b.step(4)
+            got = worlds[0].read()
+        with lenia.World(ctx, d[0]) as w:
+            w.reset(0)
+            w.frame(0, 1)
+            w.step(4)
+            w.set_params(d2)
+            w.step(4)
+            assert got.tobytes() == w.read().tobytes()
+    finally:
+        for w in worlds:
+            w.close()
+
+
+def test_batch_errors(ctx):
+    import lenia
+
+    a = lenia.World(ctx, sweep_worlds(1, seed=1)[0])
+    b = lenia.World(ctx, dict(width=64, height=64, convMode=DIRECT, radius=5,
+                              kernels=[exp_shell(5)], initMode=INIT_NONE))
+    try:
+        with pytest.raises(lenia.LeniaError):
+            lenia.Batch(ctx, [a, b])  # mixed forms
+        with pytest.raises(lenia.LeniaError):
+            lenia.Batch(ctx, [a, a])  # duplicate
+        with lenia.Batch(ctx, [a]) as bt:
+            with pytest.raises(lenia.LeniaError):
+                bt.step(10, 4)  # not a multiple of the graph length
+    finally:
+        a.close()
+        b.close()
+
+
+def test_sweep_256_one_step_matches_oracle_sample(ctx):
+    """the full 256-world sweep in one batch; a sample of worlds checked against
+    the FP64 oracle after one step from a random state"""
+    import lenia
+    import oracle as O
+
+    descs = [dict(d, initMode=INIT_NONE) for d in sweep_worlds(256)]
+    worlds = [lenia.World(ctx, d) for d in descs]
+    try:
+        states = [rand_state(d["width"], d["height"], seed=i) for i, d in enumerate(descs)]
+        for w, A in zip(worlds, states):
+            w.write(A)
+        with lenia.Batch(ctx, worlds) as bt:
+            bt.step(1)
+        for i in range(0, 256, 37):
+            want = O.step(descs[i], states[i].astype(np.float64), use_fft=True)
+            mx = float(np.abs(worlds[i].read().astype(np.float64) - want).max())
+            print(f"sweep world {i} {descs[i]['width']}x{descs[i]['height']}: max-abs {mx:.2e}")
+            assert mx <= 1e-5
+    finally:
+        for w in worlds:
+            w.close()
