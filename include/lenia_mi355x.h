@@ -153,6 +153,24 @@ int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs, uint3
 /* resolved descriptor (tile shapes and FFT plans filled in) */
 int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
 
+/* ---- differentiable rollout (no counterpart in the reference; for callers
+ * that fit parameters and initial states to target fields).  FFT form,
+ * growthMode 0, no noise / jitter / inject.  Pointers are device pointers
+ * when onDevice is non-zero, else host pointers; width*height floats each. */
+typedef struct lenia_param_grads {
+    double dMu[4], dSigma[4], dWeight[4]; /* per kernel growth centre, width, mix weight */
+    double dBetas[4][4];                  /* per kernel shell peaks (RING shape) */
+    double dDt;
+} lenia_param_grads;
+
+/* T steps from A0; the world's state ends at AT */
+int lenia_rollout_forward(lenia_world *w, uint32_t T, const float *A0, float *AT, int onDevice);
+/* reverse pass of the same rollout for L with dL/dA_T = gAT: writes dL/dA0
+ * (may be NULL) and the parameter gradients.  States are recomputed from
+ * checkpoints every ckptEvery steps (0: floor(sqrt(T))). */
+int lenia_rollout_backward(lenia_world *w, uint32_t T, uint32_t ckptEvery, const float *A0,
+                           const float *gAT, float *gA0, lenia_param_grads *grads, int onDevice);
+
 /* ---- batch: many worlds stepped together (replaces the one-variant-at-a-time
  * loop of Apps/LifePresetLab/main.cpp:718-806).  Worlds keep their own
  * buffers; a batch holds a world table and launch tiles over all of them.  All

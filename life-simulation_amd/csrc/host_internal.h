@@ -38,6 +38,7 @@ struct lenia_world {
     hipGraphExec_t graphExec = nullptr;     // captured step sequence
     bool graphExtras = false;
     uint32_t version = 0;  // bumped on every descriptor upload (batches re-read)
+    struct AdjWorkspace *adj = nullptr;  // rollout-gradient buffers, made on first use
     float *F = nullptr, *G = nullptr, *S = nullptr, *twW = nullptr, *twH = nullptr;
     // direct form
     int half = 0;
@@ -91,6 +92,7 @@ int world_apply_init(lenia_world *w);
 int world_upload_desc(lenia_world *w);
 int world_launch_steps(lenia_world *w, uint32_t n);
 void world_drop_graph(lenia_world *w);
+void world_free_adjoint(lenia_world *w);
 
 uint32_t pcg_hash_host(uint32_t v);
 uint32_t module_seed(uint32_t sceneSeed);

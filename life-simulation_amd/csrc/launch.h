@@ -28,6 +28,17 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
                          hipStream_t s);
 void direct_tile_shape(int *tx, int *ty);
 
+// adjoint (k_adjoint.hip)
+hipError_t launch_adj_rows_fwd(const AdjArgs &a, int T, const float *in, size_t fieldStride,
+                               float *out, int nf, hipStream_t s);
+hipError_t launch_adj_rows_inv(const AdjArgs &a, int T, const float *G, float *out, int ns,
+                               const float *addend, hipStream_t s);
+hipError_t launch_adj_cols(const AdjArgs &a, int T, bool adj, const float *Fin, float *Ahat,
+                           float *G, float *betaPart, hipStream_t s);
+hipError_t launch_adj_cells(const AdjArgs &a, const float *A, const float *U, const float *g,
+                            float *gpOut, float *GU, float *part, hipStream_t s);
+hipError_t launch_adj_reduce(const float *part, int nBlocks, int nv, double *acc, hipStream_t s);
+
 // init and clock (k_init.hip)
 hipError_t launch_init_patch(float *A, int W, int H, uint32_t seed, int x0, int y0, int pw,
                              int ph, hipStream_t s);

@@ -425,11 +425,14 @@ static int build_kernels_fft(lenia_world *w) {
 // fits in 64 KB of LDS.
 static constexpr size_t kLdsMax = 65536;
 
+// The swizzle sw(i) permutes indices inside aligned blocks of 32, so every
+// swizzled region is sized up to a multiple of 32 entries.
+static inline int round32(int n) { return (n + 31) & ~31; }
 size_t lenia::lds_rows(int W, int P, int K, bool kpar) {
-    return (size_t)(W + (kpar ? K : 1) * P * W) * sizeof(float2);
+    return (size_t)(W + round32((kpar ? K : 1) * P * W)) * sizeof(float2);
 }
 size_t lenia::lds_cols(int H, int C, int K, bool kpar) {
-    return (size_t)(H + (kpar ? 1 + K : 1) * C * H + (kpar ? 32 : 0)) * sizeof(float2);
+    return (size_t)(H + round32(C * H) + (kpar ? round32(K * C * H) : 0)) * sizeof(float2);
 }
 
 // does (nt transforms per tile, T threads, side-by-side kernels or not) fit
@@ -578,6 +581,7 @@ int lenia::world_upload_desc(lenia_world *w) {
 static void world_release(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
     world_drop_graph(w);
+    world_free_adjoint(w);
     void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF,
                     w->clock, w->done, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
     for (void *p : ptrs) dev_free(ctx, p);
@@ -892,6 +896,7 @@ extern "C" int lenia_world_set_params(lenia_world *w, const lenia_world_desc *de
     w->desc.colThreads = keepCT;
     w->g = growth_of(desc, w->moduleSeed);
     world_drop_graph(w);
+    if (shapes) world_free_adjoint(w);
     if (shapes) {
         rc = w->isFft ? build_kernels_fft(w) : build_kernels_direct(w);
         if (rc) return rc;

@@ -96,4 +96,15 @@ struct DirectWorldDev {
     StepClock *clock;
 };
 
+// arguments of the adjoint kernels (passed by value)
+struct AdjArgs {
+    int32_t W, H, K, nShells;
+    int32_t shellK[16];  // kernel of each (k, j) shell pair
+    FftPlan planW, planH;
+    GrowthParams g;
+    const float *twW, *twH;  // stage twiddle tables of the world's plans
+    const float *S;          // [K][Wh][H] kernel spectra / (W H)
+    const float *D;          // [nShells][Wh][H] shell-gradient spectra
+};
+
 }  // namespace lenia

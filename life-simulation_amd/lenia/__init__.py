@@ -118,6 +118,23 @@ class World:
         check(_abi.lib().lenia_world_pass_times(self._h, n, out, 4, C.byref(npass)))
         return [out[i] for i in range(npass.value)]
 
+    def rollout(self, A0, T: int) -> np.ndarray:
+        """T steps from A0 (host arrays); returns A_T"""
+        a = np.ascontiguousarray(A0, dtype=np.float32)
+        out = np.empty_like(a)
+        check(_abi.lib().lenia_rollout_forward(self._h, T, a.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def rollout_grad(self, A0, T: int, gAT, ckpt_every: int = 0) -> dict:
+        """reverse pass of rollout(A0, T) for dL/dA_T = gAT (host arrays)"""
+        a = np.ascontiguousarray(A0, dtype=np.float32)
+        g = np.ascontiguousarray(gAT, dtype=np.float32)
+        gA0 = np.empty_like(a)
+        pg = _abi.ParamGrads()
+        check(_abi.lib().lenia_rollout_backward(self._h, T, ckpt_every, a.ctypes.data,
+                                                g.ctypes.data, gA0.ctypes.data, C.byref(pg), 0))
+        return grads_dict(pg, len(self.desc["kernels"]), gA0)
+
     def resolved(self) -> dict:
         d = _abi.WorldDesc()
         check(_abi.lib().lenia_world_get_desc(self._h, C.byref(d)))
@@ -135,6 +152,16 @@ class World:
 
     def __exit__(self, *a):
         self.close()
+
+
+def grads_dict(pg, K: int, gA0=None) -> dict:
+    out = {"mu": np.array(pg.dMu[:K]), "sigma": np.array(pg.dSigma[:K]),
+           "weight": np.array(pg.dWeight[:K]),
+           "betas": np.array([[pg.dBetas[k][j] for j in range(4)] for k in range(K)]),
+           "dt": pg.dDt}
+    if gA0 is not None:
+        out["A0"] = gA0
+    return out
 
 
 class Batch:

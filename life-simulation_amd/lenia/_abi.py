@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "liblenia_mi355x.so")
@@ -34,6 +35,7 @@ EXPORTS = [
     "lenia_world_write_state", "lenia_world_state_device", "lenia_world_potential",
     "lenia_world_get_desc", "lenia_world_pass_times",
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
+    "lenia_rollout_forward", "lenia_rollout_backward",
 ]
 
 
@@ -82,6 +84,16 @@ class WorldDesc(C.Structure):
     ]
 
 
+class ParamGrads(C.Structure):
+    _fields_ = [
+        ("dMu", C.c_double * 4),
+        ("dSigma", C.c_double * 4),
+        ("dWeight", C.c_double * 4),
+        ("dBetas", (C.c_double * 4) * 4),
+        ("dDt", C.c_double),
+    ]
+
+
 class LeniaError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"lenia error {code}: {msg}")
@@ -89,15 +101,22 @@ class LeniaError(RuntimeError):
 
 
 _lib = None
+# True if torch was imported when the library was loaded.  torch bundles its
+# own libamdhip64 (soname libamdhip64.so.7): loaded after torch, the library
+# binds to that runtime; loaded before, it binds to /opt/rocm's, and a later
+# torch import would bring a second HIP runtime into the process, whose device
+# pointers the library cannot use.
+loaded_after_torch = False
 
 
 def lib():
     """Load the library (raises if it has not been built)."""
-    global _lib
+    global _lib, loaded_after_torch
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    loaded_after_torch = "torch" in sys.modules
     L = C.CDLL(LIB_PATH)
     vp, i32, u32, sz = C.c_void_p, C.c_int, C.c_uint32, C.c_size_t
     fp = C.POINTER(C.c_float)
@@ -126,6 +145,8 @@ def lib():
         "lenia_batch_create": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
         "lenia_batch_destroy": (i32, [vp]),
         "lenia_batch_step": (i32, [vp, u32, u32]),
+        "lenia_rollout_forward": (i32, [vp, u32, vp, vp, i32]),
+        "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -59,11 +59,16 @@ def config2(conv=DIRECT):
 
 
 def config4(conv=FFT):
-    """256x256, two ring kernels R=16"""
+    """the differentiable-rollout world: 256x256, two ring kernels R=16 with
+    growth (mu, sigma) = (0.30, 0.05) and (0.40, 0.07), weights .6/.4, full-frame
+    seeded noise; from noise the field keeps 60-70% of its cells strictly
+    inside (0, 1) for the first 20 steps, so gradients flow through most cells
+    (the sharper (0.15, 0.017) growth of the one-kernel worlds empties a noise
+    field within 20 steps)"""
     return dict(width=256, height=256, convMode=conv, growthMode=0, dt=0.1, radius=16,
-                kernels=[ring(16, [1.0], 0.15, 0.017, 0.6),
-                         ring(16, [1.0, 0.5], 0.25, 0.03, 0.4)],
-                initMode=INIT_PATCH, initRect=centred(256, 256, 128, 128))
+                kernels=[ring(16, [1.0], 0.30, 0.05, 0.6),
+                         ring(16, [1.0, 0.5], 0.40, 0.07, 0.4)],
+                initMode=INIT_PATCH, initRect=(0, 0, 256, 256))
 
 
 SWEEP_SIDES = [64, 96, 128, 192, 256, 384, 512]
