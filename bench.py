@@ -112,6 +112,81 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
             w.close()
 
 
+def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
+    """differentiable-rollout world (256x256, two ring kernels R=16): wall time
+    of the T-step forward rollout and of its reverse pass (checkpoints every
+    `ckpt` steps, recompute included), host arrays in and out, best of reps"""
+    import numpy as np
+
+    import lenia
+    from lenia_cases import INIT_NONE, config4
+
+    desc = config4()
+    with lenia.World(ctx, desc) as w0:
+        w0.reset(77)
+        w0.frame(0)
+        A0 = w0.read()
+    rng = np.random.default_rng(5)
+    gT = rng.standard_normal(A0.shape).astype(np.float32)
+    with lenia.World(ctx, dict(desc, initMode=INIT_NONE)) as w:
+        w.rollout(A0, T)
+        w.rollout_grad(A0, T, gT, ckpt)  # workspace, graphs, spectra
+        fwd, bwd = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            w.rollout(A0, T)
+            t1 = time.perf_counter()
+            w.rollout_grad(A0, T, gT, ckpt)
+            t2 = time.perf_counter()
+            fwd.append(t1 - t0)
+            bwd.append(t2 - t1)
+    f, b = min(fwd) * 1e3, min(bwd) * 1e3
+    return {"world": "256x256 K=2 R=16 FFT", "T": T, "ckpt_every": ckpt,
+            "forward_ms": round(f, 4), "backward_ms": round(b, 4),
+            "forward_plus_adjoint_over_forward": round((f + b) / f, 3),
+            "mcell_steps_per_s_forward": round(256 * 256 * T / f / 1e3, 1)}
+
+
+def copy_calibration(mib: int, reps: int = 10) -> dict:
+    """a plain streaming kernel of known size (torch.neg over `mib` MiB of fp32:
+    reads and writes mib MiB each) whose FETCH_SIZE / WRITE_SIZE counters scale
+    the counter readings of the Lenia kernels to bytes"""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    return {"calibration": "torch.neg fp32", "bytes_read": n * 4, "bytes_written": n * 4,
+            "launches": reps}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fft_rows_lenia_fft_1024x512_K3_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), or None"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    vals = {}
+    for line in text.splitlines():
+        if ":" in line and line.strip().startswith("- "):
+            k, v = line.strip()[2:].split(":", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("kernel") != kernel:
+        return None
+    try:
+        return int(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,7 +195,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the 256-world batched sweep")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the rollout-gradient leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain fp32 streaming kernel over MIB MiB (counter scaling)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        print(json.dumps(copy_calibration(args.copy_calibration)))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(worlds shard across GPUs with no communication; see the sharding notes)",
@@ -148,12 +229,14 @@ def main() -> int:
         tiles = w.resolved()
         A = w.read()
         batched = None if args.no_sweep else sweep_bench(ctx)
+        adjoint = None if args.no_adjoint else adjoint_bench(ctx)
     sec = gpu_ms / 1e3
     value = cells * args.steps / sec / 1e6
     bpc = fft_bytes_per_cell(W, K)
     names = ["cols", "rows"]
     per = {names[i]: passes[i] for i in range(len(passes))}
     dom = max(per, key=per.get)
+    dom_kernel = "k_fft_rows" if dom == "rows" else "k_fft_cols"
     achieved = bpc[dom] * cells / (per[dom] * 1e-3) / 1e9
     step_gbs = bpc["step"] * cells * args.steps / sec / 1e9
     out = {
@@ -176,9 +259,11 @@ def main() -> int:
                    "rowPairsPerTile": tiles["rowPairsPerTile"],
                    "colsPerTile": tiles["colsPerTile"]},
         "passes_ms": {k: round(v, 5) for k, v in per.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_fft_rows" if dom == "rows" else "k_fft_cols",
+        "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom_kernel),
+                     "algorithmic_bytes_per_launch": int(round(bpc[dom] * cells)),
                      "algorithmic_bytes_per_cell": round(bpc[dom], 3),
                      "step_achieved_gbs": round(step_gbs, 1),
                      "step_bytes_per_cell": round(bpc["step"], 3)},
@@ -186,6 +271,8 @@ def main() -> int:
     }
     if batched is not None:
         out["batched"] = batched
+    if adjoint is not None:
+        out["adjoint"] = adjoint
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, args.cpu_seconds)
     print(json.dumps(out))

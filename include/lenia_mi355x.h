@@ -144,7 +144,7 @@ int lenia_world_state_device(lenia_world *w, float **dev);
 /* potential u_k = K_k (x) A of the current state (the potential_k textures of
  * Lenia.cpp:611-616), width*height floats */
 int lenia_world_potential(lenia_world *w, uint32_t k, float *host, size_t count);
-/* runs nSteps steps with GPU timestamps between passes and writes the mean
+/* runs nSteps steps with GPU timestamps around each kernel and writes the mean
  * duration in ms of each pass (FFT form: [column pass, row pass]; direct
  * forms: [step]) -- the per-pass GPUTimer figures LifeBench reports
  * (LifeCore/Metal/GPUTimer.mm:28-94, Apps/LifeBench/main.cpp:112-133) */

@@ -85,7 +85,7 @@ size_t lds_cols(int H, int C, int K, bool kpar);
 bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
 bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
                 int &ntOut, int &TOut, bool &kparOut);
-hipError_t enqueue_step(lenia_world *w, hipEvent_t mid);
+hipError_t enqueue_step(lenia_world *w, const hipEvent_t *ev);
 int world_ensure_spectrum(lenia_world *w);
 int world_apply_init(lenia_world *w);
 

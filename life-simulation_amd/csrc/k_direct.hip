@@ -115,11 +115,11 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
                          int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
                          hipStream_t s) {
     if (mode == 0)
-        hipLaunchKernelGGL(k_direct<0>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
-                           nWorlds, kSel, out);
+        launch_timed(k_direct<0>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
+                     nWorlds, kSel, out);
     else
-        hipLaunchKernelGGL(k_direct<1>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
-                           nWorlds, kSel, out);
+        launch_timed(k_direct<1>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
+                     nWorlds, kSel, out);
     return hipGetLastError();
 }
 

@@ -490,14 +490,14 @@ static void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, 
                         int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
                         hipStream_t s) {
     if (mode == 0)
-        hipLaunchKernelGGL((k_fft_rows<0, T, false, false>), dim3(grid), dim3(T), lds, s, worlds,
-                           tiles, nTiles, done, nWorlds, kSel, out);
+        launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, done, nWorlds, kSel, out);
     else if (mode == 1)
-        hipLaunchKernelGGL((k_fft_rows<1, T, EX, KP>), dim3(grid), dim3(T), lds, s, worlds, tiles,
-                           nTiles, done, nWorlds, kSel, out);
+        launch_timed(k_fft_rows<1, T, EX, KP>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
+                     done, nWorlds, kSel, out);
     else
-        hipLaunchKernelGGL((k_fft_rows<2, T, false, true>), dim3(grid), dim3(T), lds, s, worlds,
-                           tiles, nTiles, done, nWorlds, kSel, out);
+        launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, done, nWorlds, kSel, out);
 }
 
 template <int T>
@@ -531,9 +531,9 @@ template <int T>
 static void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
                           int grid, size_t lds, hipStream_t s) {
     if (kpar)
-        hipLaunchKernelGGL((k_fft_cols<T, true>), dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
     else
-        hipLaunchKernelGGL((k_fft_cols<T, false>), dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
 }
 
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,

@@ -1,12 +1,32 @@
 // Host-side launchers of the HIP kernels (defined in the .hip files).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "lenia_types.h"
 
 namespace lenia {
+
+// Kernel timestamps.  When `start` is set, the next step-kernel launch on this
+// thread records the two events around the kernel itself
+// (hipExtLaunchKernelGGL) and clears them; otherwise launches are plain.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchEvents &launch_events();
+
+template <class K, class... A>
+inline void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, A... args) {
+    LaunchEvents &ev = launch_events();
+    if (ev.start) {
+        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)lds, s, ev.start, ev.stop, 0u, args...);
+        ev = LaunchEvents();
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+    }
+}
 
 // FFT path (k_fft.hip).  mode 0: forward row FFT of the state; 1: full step
 // row pass; 2: potential of kernel kSel into `out`.

@@ -749,21 +749,29 @@ int lenia::world_ensure_spectrum(lenia_world *w) {
 
 // the kernels of one step on the context stream; with `mid` set, an event is
 // recorded between the column and row passes (per-pass timing)
-hipError_t lenia::enqueue_step(lenia_world *w, hipEvent_t mid) {
+static thread_local LaunchEvents tl_launch_events;
+LaunchEvents &lenia::launch_events() { return tl_launch_events; }
+
+// one step's kernels; with `ev`, kernel p of the step records ev[2p] and
+// ev[2p+1] around itself
+hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev) {
     lenia_ctx *ctx = w->ctx;
     hipError_t e;
     if (w->isFft) {
+        if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles,
                             w->nColTiles, w->nColTiles, w->ldsCols, ctx->stream);
-        if (e == hipSuccess && mid) e = hipEventRecord(mid, ctx->stream);
+        if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
                                 (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                                 w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
     } else {
+        if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                           w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
     }
+    launch_events() = LaunchEvents();
     return e;
 }
 
@@ -993,38 +1001,9 @@ uint32_t lenia::module_seed(uint32_t sceneSeed) {
 
 // per-pass GPU timing of nSteps steps (GPUTimer per-pass timestamps,
 // LifeCore/Metal/GPUTimer.mm:28-94, averaged the way LifeBench reports them,
-// Apps/LifeBench/main.cpp:112-133).  Events are recorded on the context stream
-// between passes, so each figure is one kernel's duration.
-// times `body` (n repetitions captured in one graph, so the GPU runs them back
-// to back with no host gaps) with events around the graph launch
-template <class F>
-static hipError_t time_graph(lenia_ctx *ctx, uint32_t n, F body, double *ms) {
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return e;
-    hipError_t eb = hipSuccess;
-    for (uint32_t i = 0; i < n && eb == hipSuccess; ++i) eb = body();
-    e = hipStreamEndCapture(ctx->stream, &graph);
-    if (eb != hipSuccess) e = eb;
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
-    if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
-    if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
-    float f = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&f, ctx->ev0, ctx->ev1);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    *ms = f;
-    return e;
-}
-
-// Per-pass GPU time (GPUTimer per-pass timestamps, LifeCore/Metal/GPUTimer.mm:
-// 28-94, averaged as LifeBench reports them, Apps/LifeBench/main.cpp:112-133).
-// FFT form: the column pass alone is a pure function of the spectrum F, so
-// n repetitions of it are timed by themselves; then n full steps are timed
-// and the row pass is the difference.  Direct forms: one pass per step.
+// Apps/LifeBench/main.cpp:112-133).  Every kernel of every timed step carries
+// its own start/stop timestamps (hipExtLaunchKernelGGL), so each figure is the
+// mean duration of that kernel alone, launch gaps excluded.
 extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs,
                                       uint32_t maxPasses, uint32_t *nPasses) {
     if (!w || !outMs || !nPasses || nSteps == 0)
@@ -1039,25 +1018,31 @@ extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *o
     }
     int rc = world_ensure_spectrum(w);
     if (rc) return rc;
-    double colsMs = 0.0, stepMs = 0.0;
+    constexpr uint32_t kChunk = 64;  // steps between event read-backs
+    std::vector<hipEvent_t> ev(4 * kChunk, nullptr);
     hipError_t e = hipSuccess;
-    if (w->isFft) {
-        e = time_graph(ctx, nSteps, [&]() {
-            return launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc,
-                                   w->colTiles, w->nColTiles, w->nColTiles, w->ldsCols,
-                                   ctx->stream);
-        }, &colsMs);
+    for (auto &x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    double sum[2] = {0.0, 0.0};
+    for (uint32_t done = 0; done < nSteps && e == hipSuccess;) {
+        const uint32_t m = std::min(kChunk, nSteps - done);
+        uint32_t launched = 0;
+        for (; launched < m && e == hipSuccess; ++launched) e = enqueue_step(w, &ev[4 * launched]);
+        w->hostSteps += launched;
+        w->frameNext += launched;
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        for (uint32_t i = 0; i < m && e == hipSuccess; ++i)
+            for (uint32_t p = 0; p < np && e == hipSuccess; ++p) {
+                float f = 0.f;
+                e = hipEventElapsedTime(&f, ev[4 * i + 2 * p], ev[4 * i + 2 * p + 1]);
+                sum[p] += f;
+            }
+        done += m;
     }
-    if (e == hipSuccess) e = time_graph(ctx, nSteps, [&]() { return enqueue_step(w, nullptr); }, &stepMs);
+    for (auto x : ev)
+        if (x) (void)hipEventDestroy(x);
     if (e != hipSuccess) return hip_fail(e, "pass timing");
-    w->hostSteps += nSteps;
-    w->frameNext += nSteps;
-    if (w->isFft) {
-        outMs[0] = colsMs / nSteps;
-        outMs[1] = (stepMs - colsMs) / nSteps;
-    } else {
-        outMs[0] = stepMs / nSteps;
-    }
+    for (uint32_t p = 0; p < np; ++p) outMs[p] = sum[p] / nSteps;
     *nPasses = np;
     return LENIA_OK;
 }
