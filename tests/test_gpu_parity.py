@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from lenia_cases import (BF16, DIRECT, FFT, INIT_NONE, config0, config1, config4, exp_shell,
+from lenia_cases import (BF16, DIRECT, FFT, INIT_NOISE, INIT_NONE, config0, config1, config4, exp_shell,
                          gauss_shell, rand_state, ring)
 
 pytestmark = pytest.mark.gpu
@@ -151,6 +151,25 @@ def test_init_patch_bit_exact(ctx, conv):
     x0, y0, pw, ph = d["initRect"]
     want = O.init_patch(256, 256, m, x0, y0, pw, ph)
     assert np.array_equal(got.astype(np.float64), want)
+
+
+@pytest.mark.parametrize("W,H,cov,scale", [(256, 256, 0.4, 24.0), (200, 120, 0.6, 13.5)])
+def test_init_value_noise_matches_oracle(ctx, W, H, cov, scale):
+    """the value-noise soup (leniaInit) against the oracle's FP64 restatement;
+    FP32 interpolation rounding only (the restatement itself is parity
+    unpinned against the reference, whose Metal FP32 mix/smoothstep cannot run
+    here)"""
+    d = dict(config4(conv=DIRECT), width=W, height=H, dt=0.0, initMode=INIT_NOISE,
+             initCoverage=cov, initScale=scale)
+    with make(ctx, d) as w:
+        w.reset(31337)
+        w.frame(0, 1)
+        got = w.read()
+    want = O.init_value_noise(W, H, O.module_seed(31337), cov, scale)
+    mx, rms = err(got, want)
+    print(f"value-noise init {W}x{H}: max-abs {mx:.2e} rms {rms:.2e}")
+    assert mx <= 1e-5
+    assert 0.0 < float(got.mean()) < 1.0
 
 
 @pytest.mark.parametrize("conv", [DIRECT, FFT])
