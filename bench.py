@@ -147,6 +147,53 @@ def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
             "mcell_steps_per_s_forward": round(256 * 256 * T / f / 1e3, 1)}
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak, published
+VALU_FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak, published
+
+
+def direct_forms_bench(ctx, steps: int = 200) -> dict:
+    """1280x720, one exp-bump kernel R=32, full-frame seeded noise: per-step
+    kernel time of the FP32 vector direct form, the BF16 MFMA forms (plain and
+    split) and the FFT form.  Algorithmic flops = 2 x nnz per cell-step, nnz =
+    #{(dx, dy): 0 < dx^2 + dy^2 < R^2} (the exp-bump core vanishes at r = 0 and
+    r >= 1); executed MFMA flops include the band's zeros and the split's
+    extra products."""
+    import numpy as np
+
+    import lenia
+    from lenia_cases import BF16, BF16X3, DIRECT, FFT, config2
+
+    R = 32
+    d = np.arange(-R, R + 1)
+    r2 = d[:, None] ** 2 + d[None, :] ** 2
+    nnz = int(np.count_nonzero((r2 > 0) & (r2 < R * R)))
+    ksteps = (32 + 2 * R + 15) // 16
+    out = {"world": "1280x720 K=1 exp-bump R=32", "nnz": nnz, "forms": {}}
+    for conv, name, mfmas in ((DIRECT, "fp32_direct", 0), (BF16, "bf16_mfma", 1),
+                              (BF16X3, "bf16x3_mfma", 4), (FFT, "fp32_fft", 0)):
+        desc = config2(conv=conv)
+        cells = desc["width"] * desc["height"]
+        with lenia.World(ctx, desc) as w:
+            w.reset(7777)
+            w.frame(0, 1)
+            w.step(10)
+            ms = sum(w.pass_times(steps))
+        rec = {"us_per_step": round(ms * 1e3, 3),
+               "mcell_steps_per_s": round(cells / (ms * 1e-3) / 1e6, 1)}
+        alg_tflops = 2.0 * nnz * cells / (ms * 1e-3) / 1e12
+        if conv == DIRECT:
+            rec["algorithmic_tflops"] = round(alg_tflops, 2)
+            rec["valu_fp32_frac"] = round(alg_tflops / VALU_FP32_PEAK_TFLOPS, 4)
+        elif mfmas:
+            exe = 2.0 * 16 * ksteps * (2 * R + 1) * mfmas * cells / (ms * 1e-3) / 1e12
+            rec["algorithmic_tflops"] = round(alg_tflops, 2)
+            rec["executed_mfma_tflops"] = round(exe, 2)
+            rec["mfma_util_algorithmic"] = round(alg_tflops / MFMA_BF16_PEAK_TFLOPS, 4)
+            rec["mfma_util_executed"] = round(exe / MFMA_BF16_PEAK_TFLOPS, 4)
+        out["forms"][name] = rec
+    return out
+
+
 def copy_calibration(mib: int, reps: int = 10) -> dict:
     """a plain streaming kernel of known size (torch.neg over `mib` MiB of fp32:
     reads and writes mib MiB each) whose FETCH_SIZE / WRITE_SIZE counters scale
@@ -196,6 +243,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the 256-world batched sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the rollout-gradient leg")
+    ap.add_argument("--no-direct", action="store_true", help="skip the 1280x720 direct-forms leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming kernel over MIB MiB (counter scaling)")
     args = ap.parse_args()
@@ -230,6 +278,7 @@ def main() -> int:
         A = w.read()
         batched = None if args.no_sweep else sweep_bench(ctx)
         adjoint = None if args.no_adjoint else adjoint_bench(ctx)
+        direct = None if args.no_direct else direct_forms_bench(ctx)
     sec = gpu_ms / 1e3
     value = cells * args.steps / sec / 1e6
     bpc = fft_bytes_per_cell(W, K)
@@ -273,6 +322,8 @@ def main() -> int:
         out["batched"] = batched
     if adjoint is not None:
         out["adjoint"] = adjoint
+    if direct is not None:
+        out["direct_forms"] = direct
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, args.cpu_seconds)
     print(json.dumps(out))

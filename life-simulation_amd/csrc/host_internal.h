@@ -40,10 +40,14 @@ struct lenia_world {
     uint32_t version = 0;  // bumped on every descriptor upload (batches re-read)
     struct AdjWorkspace *adj = nullptr;  // rollout-gradient buffers, made on first use
     float *F = nullptr, *G = nullptr, *S = nullptr, *twW = nullptr, *twH = nullptr;
-    // direct form
+    // direct forms
     int half = 0;
     float *winF = nullptr;
     size_t winBytes = 0;
+    uint16_t *toepHi = nullptr, *toepMid = nullptr, *toepLo = nullptr;  // MFMA forms: Toeplitz fragments
+    size_t toepBytes = 0;
+    int ksteps = 0;
+    bool split() const { return desc.convMode == LENIA_CONV_DIRECT_BF16X3; }
     // both
     float *A0 = nullptr, *A1 = nullptr;
     float *scratchOut = nullptr;
@@ -52,6 +56,7 @@ struct lenia_world {
     void *devDesc = nullptr;
     lenia::FftWorldDev fdev{};
     lenia::DirectWorldDev ddev{};
+    lenia::MfmaWorldDev mdev{};
     int2 *rowTiles = nullptr, *colTiles = nullptr;
     int nRowTiles = 0, nColTiles = 0;
     size_t ldsRows = 0, ldsCols = 0;

@@ -1,0 +1,312 @@
+// Direct-convolution form of the Lenia step on the gfx950 matrix cores
+// (BF16 MFMA, FP32 accumulation), plain and split-BF16.
+//
+// Same operator as the reference's leniaStep (Shaders/Field/Lenia.metal:
+// 63-128, extended to K kernels by leniaGrowthMulti :161-232), recast as a
+// banded contraction.  For one window row dy and a 32 x 32 output block at
+// (y0, x0):
+//
+//   u[y0+n][x0+m] += sum_j T_dy[m][j] * F_dy[j][n]
+//   F_dy[j][n]  = A[y0+n+dy-h][x0-h+j]                     (the state, j < J)
+//   T_dy[m][j]  = K[dy][j-m] if 0 <= j-m <= 2h, else 0     (banded Toeplitz)
+//
+// one 32x32xJ matrix product per window row, J = 16 KS >= 32 + 2h.  T is the
+// MFMA A operand (rows = output columns x), F the B operand (columns = output
+// rows y): the state operand is an aligned 8-element run of one LDS row per
+// lane, and the shifted Toeplitz fragments -- the same for every tile -- are
+// precomputed on the host in register order (64 lanes x 8 bf16 per k-step)
+// and streamed from L2.
+//
+// Work split: a workgroup owns a 32 (x) x 32 NB (y) output tile (NB blocks
+// stacked in y).  Its 4 waves split the 2h+1 window rows into quarters; each
+// wave applies every fragment it loads to all NB blocks (NB independent
+// accumulation chains), the quarters' sums are reduced through LDS in a fixed
+// order, and wave b finishes block b.
+//
+// Split-BF16 (BF16X3): the state in two terms, F = Fhi + Flo (hi = bf16(v),
+// lo = bf16(v - hi), residual <= 2^-18 |v|), the kernel in three, T = Thi +
+// Tmid + Tlo (residual <= 2^-27 |T|), and four MFMAs per k-step:
+// Tlo*Fhi + Thi*Flo + Tmid*Fhi + Thi*Fhi.  The kernel needs the third term
+// because its split residuals are systematic (equal weights of a symmetric
+// window round the same way, so a 2^-18 residual would add up coherently);
+// the state residuals have random signs and average out over the window.
+// Accumulation is two-level: each wave sums its quarter of the window rows,
+// then the four quarter sums are added.
+#include <hip/hip_runtime.h>
+
+#include "lenia_device.h"
+#include "launch.h"
+
+namespace lenia {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMW = 4;         // waves per workgroup
+constexpr int kMT = 64 * kMW;  // threads
+constexpr int kMaxSteps = 8;   // k-steps of 16 per window row (J <= 128, h <= 48)
+constexpr int kStage = 16;     // state-tile loads in flight per thread
+constexpr int kRedFloats = 2 * 64 * 16;  // reduction buffer: two waves' block totals
+
+// output blocks (32 rows each) per workgroup
+template <bool SPLIT>
+constexpr int blocks_y() { return SPLIT ? 2 : 4; }
+
+// round-to-nearest-even FP32 -> BF16 bits (finite inputs)
+__device__ __forceinline__ unsigned short bf16_bits(float v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(unsigned short b) {
+    return __uint_as_float((uint32_t)b << 16);
+}
+
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+template <bool SPLIT, int KS>
+struct Frags {
+    u32x4 hi[KS];
+    u32x4 mid[SPLIT ? KS : 1];
+    u32x4 lo[SPLIT ? KS : 1];
+};
+
+// Toeplitz fragments of window row dy of kernel k: [k][dy][s][lane] x 16 B
+template <bool SPLIT, int KS>
+__device__ __forceinline__ void load_toeplitz(Frags<SPLIT, KS> &f, const MfmaWorldDev &wd, int k,
+                                              int dy, int lane) {
+    const size_t base = (((size_t)k * (2 * wd.half + 1) + dy) * KS) * 64 + lane;
+    const LENIA_GLOBAL u32x4 *hi = gp(reinterpret_cast<const u32x4 *>(wd.toepHi)) + base;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) f.hi[s] = hi[s * 64];
+    if constexpr (SPLIT) {
+        const LENIA_GLOBAL u32x4 *mid = gp(reinterpret_cast<const u32x4 *>(wd.toepMid)) + base;
+        const LENIA_GLOBAL u32x4 *lo = gp(reinterpret_cast<const u32x4 *>(wd.toepLo)) + base;
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            f.mid[s] = mid[s * 64];
+            f.lo[s] = lo[s * 64];
+        }
+    }
+}
+
+// one window row applied to the NB blocks, accumulated into total[b];
+// rowHi/rowLo point at the row of block 0
+template <bool SPLIT, int KS, int NB>
+__device__ __forceinline__ void rows_mma(f32x16 (&total)[NB], const Frags<SPLIT, KS> &f,
+                                         const unsigned short *rowHi, const unsigned short *rowLo,
+                                         int blockStride) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const bf16x8 th = as_bf16x8(f.hi[s]);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const unsigned short *ph = rowHi + b * blockStride + 16 * s;
+            const bf16x8 fh = as_bf16x8(*reinterpret_cast<const u32x4 *>(ph));
+            if constexpr (SPLIT) {
+                const unsigned short *pl = rowLo + b * blockStride + 16 * s;
+                const bf16x8 fl = as_bf16x8(*reinterpret_cast<const u32x4 *>(pl));
+                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.lo[s]), fh, total[b], 0, 0, 0);
+                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fl, total[b], 0, 0, 0);
+                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.mid[s]), fh, total[b], 0, 0, 0);
+            }
+            total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fh, total[b], 0, 0, 0);
+        }
+    }
+}
+
+// MODE 0: full step (src -> dst by clock parity); MODE 1: potential of kernel
+// kSel of the current state into `out`
+template <int MODE, bool SPLIT, int KS>
+__global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ worlds,
+                                              const int2 *__restrict__ tiles, int nTiles,
+                                              uint32_t *done, int nWorlds, int kSel,
+                                              float *__restrict__ out) {
+    constexpr int NB = blocks_y<SPLIT>();
+    extern __shared__ unsigned short smemMf[];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
+        const int2 tile = tiles[ti];
+        const MfmaWorldDev &wd = worlds[tile.x];
+        const int W = wd.W, H = wd.H, h = wd.half;
+        const int x0 = (tile.y % wd.tilesX) * 32;
+        const int y0 = (tile.y / wd.tilesX) * (32 * NB);
+        const int rows = 32 * NB + 2 * h, cols = 16 * KS, pitch = wd.pitch;
+        const int plane = rows * pitch;
+        unsigned short *tHi = smemMf;
+        unsigned short *tLo = smemMf + plane;
+        float *red = reinterpret_cast<float *>(smemMf + (SPLIT ? 2 : 1) * plane);
+        const StepClock ck = read_clock(wd.clock);
+        const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
+        LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
+        // state tile: rows y0-h .. y0+32NB-1+h, columns x0-h .. x0-h+cols-1
+        // (toroidal); kStage loads in flight per thread before any is stored
+        const int nst = rows * cols;
+        for (int base = 0; base < nst; base += kMT * kStage) {
+            float v[kStage];
+            int at[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int i = base + u * kMT + tid;
+                at[u] = -1;
+                if (i < nst) {
+                    const int ly = i / cols, lx = i - ly * cols;
+                    int gy = (y0 - h + ly) % H, gx = (x0 - h + lx) % W;
+                    if (gy < 0) gy += H;
+                    if (gx < 0) gx += W;
+                    v[u] = src[(size_t)gy * W + gx];
+                    at[u] = ly * pitch + lx;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                if (at[u] >= 0) {
+                    const unsigned short hb = bf16_bits(v[u]);
+                    tHi[at[u]] = hb;
+                    if constexpr (SPLIT) tLo[at[u]] = bf16_bits(v[u] - bf16_value(hb));
+                }
+            }
+        }
+        __syncthreads();
+
+        const GrowthParams &g = wd.g;
+        const uint32_t seed = clock_seed(ck, g.moduleSeed);
+        const int n = lane & 31, grp = lane >> 5;
+        const int nrows = 2 * h + 1;
+        const int r0 = (nrows * wave) / kMW, r1 = (nrows * (wave + 1)) / kMW;
+        const unsigned short *colHi = tHi + n * pitch + 8 * grp;
+        const unsigned short *colLo = tLo + n * pitch + 8 * grp;
+        // wave b < NB finishes output block b: its rows y0 + 32 b + n, columns
+        // x0 + 8 (r/4) + 4 grp + (r%4) of lane slot r
+        const int yb = y0 + 32 * wave + n;
+        float gacc[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) gacc[r] = 0.0f;
+        const int kBeg = MODE == 1 ? kSel : 0;
+        const int kEnd = MODE == 1 ? kSel + 1 : (int)g.nKernels;
+        for (int k = kBeg; k < kEnd; ++k) {
+            f32x16 total[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) total[b] = f32x16{};
+            if (r0 < r1) {
+                // two fragment sets: row dy+1 loads while row dy multiplies
+                Frags<SPLIT, KS> fa, fb;
+                load_toeplitz<SPLIT, KS>(fa, wd, k, r0, lane);
+                int dy = r0;
+                for (; dy + 1 < r1; dy += 2) {
+                    load_toeplitz<SPLIT, KS>(fb, wd, k, dy + 1, lane);
+                    rows_mma<SPLIT, KS, NB>(total, fa, colHi + dy * pitch, colLo + dy * pitch, 32 * pitch);
+                    load_toeplitz<SPLIT, KS>(fa, wd, k, min(dy + 2, r1 - 1), lane);
+                    rows_mma<SPLIT, KS, NB>(total, fb, colHi + (dy + 1) * pitch,
+                                            colLo + (dy + 1) * pitch, 32 * pitch);
+                }
+                if (dy < r1)
+                    rows_mma<SPLIT, KS, NB>(total, fa, colHi + dy * pitch, colLo + dy * pitch, 32 * pitch);
+            }
+            // fixed-order reduction of block b into wave b over the window-row
+            // quarters q = 0..3 (wave w holds quarter w): (q_b + q_b+2) + (q_b+1 + q_b+3)
+            float mine[16];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int rel = (wave - b + kMW) & (kMW - 1);  // 0 = owner
+                if (rel >= 2)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) red[((rel - 2) * 16 + r) * 64 + lane] = total[b][r];
+                __syncthreads();
+                if (rel < 2)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) total[b][r] += red[(rel * 16 + r) * 64 + lane];
+                __syncthreads();
+                if (rel == 1)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) red[r * 64 + lane] = total[b][r];
+                __syncthreads();
+                if (rel == 0)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) mine[r] = total[b][r] + red[r * 64 + lane];
+                __syncthreads();
+            }
+            if (wave < NB) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
+                    if (MODE == 1) {
+                        if (x < W && yb < H) gp(out)[(size_t)yb * W + x] = mine[r];
+                    } else if (x < W && yb < H) {
+                        float muJ = cell_jitter(g, x, yb, seed, ck.frame);
+                        gacc[r] += growth_term(g, k, mine[r], muJ);
+                    }
+                }
+            }
+        }
+        if (MODE == 0 && wave < NB) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
+                if (x < W && yb < H) {
+                    float a = src[(size_t)yb * W + x];
+                    dst[(size_t)yb * W + x] = growth_finish(g, a, gacc[r], x, yb, W, H, seed, ck.frame);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (MODE == 0) arrive_step(done, worlds, nWorlds);
+}
+
+template <int KS>
+static void mfma_launch_ks(int mode, bool split, const MfmaWorldDev *worlds, const int2 *tiles,
+                           int nTiles, int grid, size_t lds, uint32_t *done, int nWorlds, int kSel,
+                           float *out, hipStream_t s) {
+    if (mode == 0) {
+        if (split)
+            launch_timed(k_mfma<0, true, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
+                         done, nWorlds, kSel, out);
+        else
+            launch_timed(k_mfma<0, false, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
+                         done, nWorlds, kSel, out);
+    } else {
+        if (split)
+            launch_timed(k_mfma<1, true, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
+                         done, nWorlds, kSel, out);
+        else
+            launch_timed(k_mfma<1, false, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
+                         done, nWorlds, kSel, out);
+    }
+}
+
+// ksteps = ceil((32 + 2h) / 16): 2 (h = 0) .. 8 (h <= 48)
+hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,
+                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
+                       int nWorlds, int kSel, float *out, hipStream_t s) {
+    switch (ksteps) {
+        case 2: mfma_launch_ks<2>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 3: mfma_launch_ks<3>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 4: mfma_launch_ks<4>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 5: mfma_launch_ks<5>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 6: mfma_launch_ks<6>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 7: mfma_launch_ks<7>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 8: mfma_launch_ks<8>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+void mfma_tile_shape(bool split, int *tx, int *ty) {
+    *tx = 32;
+    *ty = 32 * (split ? blocks_y<true>() : blocks_y<false>());
+}
+
+int mfma_max_ksteps() { return kMaxSteps; }
+
+size_t mfma_lds_bytes(int h, bool split) {
+    const int ks = (32 + 2 * h + 15) / 16;
+    const int pitch = 16 * ks + 8;
+    const int rows = 32 * (split ? blocks_y<true>() : blocks_y<false>()) + 2 * h;
+    return (size_t)rows * pitch * sizeof(unsigned short) * (split ? 2 : 1) +
+           kRedFloats * sizeof(float);
+}
+
+}  // namespace lenia

@@ -48,6 +48,14 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
                          hipStream_t s);
 void direct_tile_shape(int *tx, int *ty);
 
+// MFMA direct forms (k_mfma.hip).  mode 0: step; 1: potential of kernel kSel.
+hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,
+                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
+                       int nWorlds, int kSel, float *out, hipStream_t s);
+void mfma_tile_shape(bool split, int *tx, int *ty);
+int mfma_max_ksteps();
+size_t mfma_lds_bytes(int h, bool split);
+
 // adjoint (k_adjoint.hip)
 hipError_t launch_adj_rows_fwd(const AdjArgs &a, int T, const float *in, size_t fieldStride,
                                float *out, int nf, hipStream_t s);

@@ -356,6 +356,67 @@ static GrowthParams growth_of(const lenia_world_desc *d, uint32_t moduleSeed) {
 }
 
 // ------------------------------------------------------------------ world setup
+// FP32 -> BF16 bits, round to nearest even (finite values), and back
+static uint16_t bf16_bits_host(float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+static float bf16_value_host(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// MFMA forms: Toeplitz fragments of every window row in MFMA register order.
+// For kernel k, window row dy, k-step s and lane l (output column m = l % 32,
+// k-group g = l / 32) the 8 values K[dy][16 s + 8 g + i - m], i < 8 (zero
+// outside the window), as three BF16 planes: hi = bf16(v), mid = bf16(v - hi),
+// lo = bf16(v - hi - mid).
+static int build_toeplitz(lenia_world *w, const std::vector<float> &all, int h) {
+    lenia_ctx *ctx = w->ctx;
+    const int K = (int)w->desc.nKernels, n = 2 * h + 1;
+    const int ks = (32 + 2 * h + 15) / 16;
+    w->ksteps = ks;
+    const size_t nfr = (size_t)K * n * ks * 64 * 8;
+    std::vector<uint16_t> hi(nfr), mid(nfr), lo(nfr);
+    for (int k = 0; k < K; ++k)
+        for (int dy = 0; dy < n; ++dy)
+            for (int s = 0; s < ks; ++s)
+                for (int l = 0; l < 64; ++l)
+                    for (int i = 0; i < 8; ++i) {
+                        const int t = 16 * s + 8 * (l / 32) + i - (l % 32);
+                        const float v = (t >= 0 && t < n) ? all[((size_t)k * n + dy) * n + t] : 0.0f;
+                        const size_t idx = ((((size_t)k * n + dy) * ks + s) * 64 + l) * 8 + i;
+                        hi[idx] = bf16_bits_host(v);
+                        const float r1 = v - bf16_value_host(hi[idx]);
+                        mid[idx] = bf16_bits_host(r1);
+                        lo[idx] = bf16_bits_host(r1 - bf16_value_host(mid[idx]));
+                    }
+    const size_t bytes = nfr * sizeof(uint16_t);
+    if (!w->toepHi || w->toepBytes != bytes) {
+        dev_free(ctx, w->toepHi);
+        dev_free(ctx, w->toepMid);
+        dev_free(ctx, w->toepLo);
+        w->toepHi = w->toepMid = w->toepLo = nullptr;
+        int rc = dev_alloc(ctx, (void **)&w->toepHi, bytes);
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->toepMid, bytes);
+        if (!rc) rc = dev_alloc(ctx, (void **)&w->toepLo, bytes);
+        if (rc) return rc;
+        w->toepBytes = bytes;
+    }
+    HIP_TRY(hipMemcpyAsync(w->toepHi, hi.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+            "upload Toeplitz fragments");
+    HIP_TRY(hipMemcpyAsync(w->toepMid, mid.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+            "upload Toeplitz fragments");
+    HIP_TRY(hipMemcpyAsync(w->toepLo, lo.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+            "upload Toeplitz fragments");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "upload Toeplitz fragments");
+    return LENIA_OK;
+}
+
 static int build_kernels_direct(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
     const lenia_world_desc &d = w->desc;
@@ -380,7 +441,7 @@ static int build_kernels_direct(lenia_world *w) {
     HIP_TRY(hipMemcpyAsync(w->winF, all.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
             "upload kernel windows");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload kernel windows");
-    return LENIA_OK;
+    return w->isMfma ? build_toeplitz(w, all, h) : LENIA_OK;
 }
 
 static int build_kernels_fft(lenia_world *w) {
@@ -549,6 +610,35 @@ int lenia::world_upload_desc(lenia_world *w) {
         w->nColTiles = f.colTiles;
         w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
         w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
+    } else if (w->isMfma) {
+        MfmaWorldDev md;
+        std::memset(&md, 0, sizeof(md));
+        int tx, ty;
+        mfma_tile_shape(w->split(), &tx, &ty);
+        md.W = w->desc.width;
+        md.H = w->desc.height;
+        md.half = w->half;
+        md.tilesX = (md.W + tx - 1) / tx;
+        md.tilesY = (md.H + ty - 1) / ty;
+        md.ksteps = w->ksteps;
+        md.tileCols = 16 * w->ksteps;
+        md.pitch = md.tileCols + 8;
+        md.g = w->g;
+        md.toepHi = w->toepHi;
+        md.toepMid = w->toepMid;
+        md.toepLo = w->toepLo;
+        md.A0 = w->A0;
+        md.A1 = w->A1;
+        md.clock = w->clock;
+        w->mdev = md;
+        HIP_TRY(hipMemcpyAsync(w->devDesc, &md, sizeof(md), hipMemcpyHostToDevice, ctx->stream),
+                "upload world descriptor");
+        std::vector<int2> t;
+        for (int i = 0; i < md.tilesX * md.tilesY; ++i) t.push_back(make_int2(0, i));
+        HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
+                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
+        w->nRowTiles = (int)t.size();
+        w->ldsRows = mfma_lds_bytes(w->half, w->split());
     } else {
         DirectWorldDev dd;
         std::memset(&dd, 0, sizeof(dd));
@@ -582,7 +672,7 @@ static void world_release(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
     world_drop_graph(w);
     world_free_adjoint(w);
-    void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF,
+    void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF, w->toepHi, w->toepMid, w->toepLo,
                     w->clock, w->done, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
     for (void *p : ptrs) dev_free(ctx, p);
 }
@@ -621,8 +711,17 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         }
     }
     if (w->isMfma) {
-        delete w;
-        return fail(LENIA_ERR_UNSUPPORTED, "MFMA direct forms are not built yet");
+        int h = 0;
+        for (uint32_t k = 0; k < desc->nKernels; ++k) h = std::max(h, kernel_half(desc->kernels[k]));
+        const int ks = (32 + 2 * h + 15) / 16;
+        const size_t lds = mfma_lds_bytes(h, w->split());
+        if (ks > mfma_max_ksteps() || lds > kLdsMax) {
+            delete w;
+            return fail(LENIA_ERR_UNSUPPORTED,
+                        "MFMA form: window half-width %d needs %zu B of LDS per workgroup (limit %zu)%s",
+                        h, lds, (size_t)kLdsMax,
+                        w->split() ? "; the plain BF16 form or the FP32 forms serve it" : "");
+        }
     }
     w->moduleSeed = 0;
     w->g = growth_of(desc, w->moduleSeed);
@@ -633,7 +732,9 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
     if (!rc && !w->isFft) rc = dev_alloc(ctx, (void **)&w->A1, cells * sizeof(float));
     if (!rc) rc = dev_alloc(ctx, (void **)&w->clock, sizeof(StepClock));
     if (!rc) rc = dev_alloc(ctx, (void **)&w->done, sizeof(uint32_t) * 4);
-    if (!rc) rc = dev_alloc(ctx, (void **)&w->devDesc, std::max(sizeof(FftWorldDev), sizeof(DirectWorldDev)));
+    if (!rc)
+        rc = dev_alloc(ctx, (void **)&w->devDesc,
+                       std::max({sizeof(FftWorldDev), sizeof(DirectWorldDev), sizeof(MfmaWorldDev)}));
     if (!rc) rc = dev_alloc(ctx, (void **)&w->scratchOut, cells * sizeof(float));
     if (w->isFft) {
         if (!rc) rc = dev_alloc(ctx, (void **)&w->F, (size_t)Wh * H * sizeof(float2));
@@ -647,7 +748,10 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         if (!rc) rc = dev_alloc(ctx, (void **)&w->colTiles, nc * sizeof(int2));
     } else {
         int tx, ty;
-        direct_tile_shape(&tx, &ty);
+        if (w->isMfma)
+            mfma_tile_shape(w->split(), &tx, &ty);
+        else
+            direct_tile_shape(&tx, &ty);
         int nt = ((W + tx - 1) / tx) * ((H + ty - 1) / ty);
         if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nt * sizeof(int2));
     }
@@ -747,8 +851,6 @@ int lenia::world_ensure_spectrum(lenia_world *w) {
     return LENIA_OK;
 }
 
-// the kernels of one step on the context stream; with `mid` set, an event is
-// recorded between the column and row passes (per-pass timing)
 static thread_local LaunchEvents tl_launch_events;
 LaunchEvents &lenia::launch_events() { return tl_launch_events; }
 
@@ -766,6 +868,10 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev) {
             e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
                                 (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                                 w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+    } else if (w->isMfma) {
+        if (ev) launch_events() = {ev[0], ev[1]};
+        e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                        w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
@@ -963,6 +1069,9 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
             e = launch_fft_rows(2, w->rowT, false, true, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                                 w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut,
                                 ctx->stream);
+    } else if (w->isMfma) {
+        e = launch_mfma(1, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+                        w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut, ctx->stream);
     } else {
         e = launch_direct(1, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                           w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut, ctx->stream);

@@ -96,6 +96,21 @@ struct DirectWorldDev {
     StepClock *clock;
 };
 
+// MFMA direct form: the direct-form world plus the precomputed Toeplitz
+// fragments of every kernel row ([k][dy][kstep][lane] x 8 bf16)
+struct MfmaWorldDev {
+    int32_t W, H;
+    int32_t half;       // window half-width (max over kernels)
+    int32_t tilesX, tilesY;
+    int32_t ksteps;     // 16-wide k-steps per window row: ceil((32 + 2 half) / 16)
+    int32_t tileCols;   // staged tile columns: 16 ksteps
+    int32_t pitch;      // LDS row pitch in bf16 (tileCols + 8)
+    GrowthParams g;
+    const uint16_t *toepHi, *toepMid, *toepLo;  // kernel = hi + mid + lo (bf16 planes)
+    float *A0, *A1;     // ping-pong states
+    StepClock *clock;
+};
+
 // arguments of the adjoint kernels (passed by value)
 struct AdjArgs {
     int32_t W, H, K, nShells;
