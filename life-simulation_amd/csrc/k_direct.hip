@@ -3,11 +3,17 @@
 // Same arithmetic as the reference's leniaStep (Shaders/Field/Lenia.metal:
 // 63-128): u = sum_{dy,dx} K[dy][dx] A[wrap(y+dy)][wrap(x+dx)], growth, clip,
 // extended to up to four kernels with the leniaGrowthMulti rule
-// (Lenia.metal:161-232).  A workgroup stages a (16+2h) x (64+2h) halo tile of
-// the toroidal state in LDS, each thread produces a column of M=4 outputs and
-// reuses every LDS value for all M outputs it touches.  Each output's sum is
-// taken per window row first, then across rows (two-level summation keeps the
-// FP32 potential within ~1e-6 of the FP64 value for 3000-tap windows).
+// (Lenia.metal:161-232).
+//
+// A 128-thread workgroup owns a 64 (x) x 16 (y) output tile and stages its
+// (16+2h) x (64+P) toroidal halo in LDS once (P = window width rounded up to
+// 8; the window rows are zero padded to P).  Each thread produces 8
+// horizontally adjacent outputs of one row.  For every window row it slides
+// a 16-value register window along the staged row, 8 taps at a time: two
+// 16-byte LDS reads and 8 weights feed 64 FMAs, so the loop is VALU-bound.
+// Each output's sum is taken per window row first, then across rows
+// (two-level summation keeps the FP32 potential within ~1e-6 of the FP64
+// value for 3000-tap windows).
 #include <hip/hip_runtime.h>
 
 #include "lenia_device.h"
@@ -15,15 +21,53 @@
 
 namespace lenia {
 
-constexpr int kDT = 256;  // threads: 64 x 4
-constexpr int kDTX = 64;
-constexpr int kDM = 4;    // outputs per thread (vertical)
-constexpr int kDTY = 4 * kDM;
-
+constexpr int kDT = 128;   // threads: 8 (x) x 16 (y)
+constexpr int kDTX = 64;   // output tile width
+constexpr int kDTY = 16;   // output tile height
+constexpr int kDOX = 8;    // outputs per thread (x)
+constexpr int kDStage = 16;
 
 __device__ __forceinline__ int wrap_i(int p, int n) {
     int r = p % n;
     return r < 0 ? r + n : r;
+}
+
+// 8 consecutive LDS floats, 16-byte aligned (two ds_read_b128)
+__device__ __forceinline__ void lds_read8(const float4 *p, float (&v)[8]) {
+    const float4 a = p[0], b = p[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+    v[4] = b.x;
+    v[5] = b.y;
+    v[6] = b.z;
+    v[7] = b.w;
+}
+
+// window weights are the same for every lane: read them through the
+// constant address space, so they arrive as scalar loads into SGPRs
+#define LENIA_CONST __attribute__((address_space(4)))
+typedef float cv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void const_read8(const LENIA_CONST cv4 *p, float (&w)[8]) {
+    const cv4 a = p[0], b = p[1];
+    w[0] = a.x;
+    w[1] = a.y;
+    w[2] = a.z;
+    w[3] = a.w;
+    w[4] = b.x;
+    w[5] = b.y;
+    w[6] = b.z;
+    w[7] = b.w;
+}
+
+// 8 taps of one window row: out[i] += w[t] * in[i + t], in = lo ++ hi
+__device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const float (&lo)[8],
+                                      const float (&hi)[8]) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) rs[i] = fmaf(w[t], (i + t < 8) ? lo[i + t] : hi[i + t - 8], rs[i]);
 }
 
 // MODE 0: full step (src -> dst by clock parity); MODE 1: potential of kernel
@@ -35,74 +79,98 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
                                                 float *__restrict__ out) {
     extern __shared__ float tileS[];
     const int tid = threadIdx.x;
-    const int tx = tid % kDTX, ty = tid / kDTX;
+    const int tx = tid & 7, ty = tid >> 3;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
         const int2 tile = tiles[ti];
         const DirectWorldDev &wd = worlds[tile.x];
-        const int W = wd.W, H = wd.H, h = wd.half, n = 2 * h + 1;
+        const int W = wd.W, H = wd.H, h = wd.half, n = 2 * h + 1, np = wd.npad;
         const int x0 = (tile.y % wd.tilesX) * kDTX;
         const int y0 = (tile.y / wd.tilesX) * kDTY;
-        const int pitch = kDTX + 2 * h, rows = kDTY + 2 * h;
+        const int rows = kDTY + 2 * h, cols = kDTX + np, pitch = cols + 4;
         const StepClock ck = read_clock(wd.clock);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
-        for (int i = tid; i < rows * pitch; i += kDT) {
-            int ly = i / pitch, lx = i - ly * pitch;
-            int gy = wrap_i(y0 - h + ly, H), gx = wrap_i(x0 - h + lx, W);
-            tileS[i] = src[gy * W + gx];
+        // halo rows y0-h .. y0+15+h, columns x0-h .. x0-h+cols-1 (toroidal;
+        // columns past x0+63+h only meet zero weights)
+        const int nst = rows * cols;
+        for (int base = 0; base < nst; base += kDT * kDStage) {
+            float v[kDStage];
+            int at[kDStage];
+#pragma unroll
+            for (int u = 0; u < kDStage; ++u) {
+                const int i = base + u * kDT + tid;
+                at[u] = -1;
+                if (i < nst) {
+                    const int ly = i / cols, lx = i - ly * cols;
+                    v[u] = src[(size_t)wrap_i(y0 - h + ly, H) * W + wrap_i(x0 - h + lx, W)];
+                    at[u] = ly * pitch + lx;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kDStage; ++u)
+                if (at[u] >= 0) tileS[at[u]] = v[u];
         }
         __syncthreads();
 
         const GrowthParams &g = wd.g;
         const uint32_t seed = clock_seed(ck, g.moduleSeed);
-        const int x = x0 + tx;
-        float acc[kDM];
+        const int y = y0 + ty, xb = x0 + kDOX * tx;
+        float gacc[kDOX];
 #pragma unroll
-        for (int j = 0; j < kDM; ++j) acc[j] = 0.0f;
+        for (int i = 0; i < kDOX; ++i) gacc[i] = 0.0f;
         const int kBeg = MODE == 1 ? kSel : 0;
         const int kEnd = MODE == 1 ? kSel + 1 : (int)g.nKernels;
+        const int chunks = np / 8;
+        const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
+        const int pitch4 = pitch / 4;
         for (int k = kBeg; k < kEnd; ++k) {
-            const LENIA_GLOBAL float *wk = gp(wd.win) + k * n * n;
-            float tot[kDM];
+            const LENIA_CONST cv4 *wk = (const LENIA_CONST cv4 *)(wd.win) + (size_t)k * n * (np / 4);
+            float tot[kDOX];
 #pragma unroll
-            for (int j = 0; j < kDM; ++j) tot[j] = 0.0f;
-            for (int r = 0; r < kDM + 2 * h; ++r) {
-                const float *trow = tileS + (ty * kDM + r) * pitch + tx;
-                float rs[kDM];
+            for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
+            for (int iy = 0; iy < n; ++iy) {
+                const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
+                const LENIA_CONST cv4 *wrow = wk + (size_t)iy * (np / 4);
+                float rs[kDOX];
 #pragma unroll
-                for (int j = 0; j < kDM; ++j) rs[j] = 0.0f;
-                for (int ix = 0; ix < n; ++ix) {
-                    float v = trow[ix];
-#pragma unroll
-                    for (int j = 0; j < kDM; ++j) {
-                        int iy = r - j;
-                        if (iy >= 0 && iy < n) rs[j] = fmaf(wk[iy * n + ix], v, rs[j]);
-                    }
+                for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
+                float va[8], vb[8], w[8];
+                lds_read8(trow, va);
+                int c = 0;
+                for (; c + 1 < chunks; c += 2) {
+                    lds_read8(trow + 2 * c + 2, vb);
+                    const_read8(wrow + 2 * c, w);
+                    taps8(rs, w, va, vb);
+                    lds_read8(trow + 2 * c + 4, va);
+                    const_read8(wrow + 2 * c + 2, w);
+                    taps8(rs, w, vb, va);
+                }
+                if (c < chunks) {
+                    lds_read8(trow + 2 * c + 2, vb);
+                    const_read8(wrow + 2 * c, w);
+                    taps8(rs, w, va, vb);
                 }
 #pragma unroll
-                for (int j = 0; j < kDM; ++j) {
-                    int iy = r - j;
-                    if (iy >= 0 && iy < n) tot[j] += rs[j];
-                }
+                for (int i = 0; i < kDOX; ++i) tot[i] += rs[i];
             }
 #pragma unroll
-            for (int j = 0; j < kDM; ++j) {
-                int y = y0 + ty * kDM + j;
+            for (int i = 0; i < kDOX; ++i) {
+                const int x = xb + i;
                 if (MODE == 1) {
-                    if (x < W && y < H) gp(out)[y * W + x] = tot[j];
+                    if (x < W && y < H) gp(out)[(size_t)y * W + x] = tot[i];
                 } else if (x < W && y < H) {
                     float muJ = cell_jitter(g, x, y, seed, ck.frame);
-                    acc[j] += growth_term(g, k, tot[j], muJ);
+                    gacc[i] += growth_term(g, k, tot[i], muJ);
                 }
             }
         }
         if (MODE == 0) {
 #pragma unroll
-            for (int j = 0; j < kDM; ++j) {
-                int y = y0 + ty * kDM + j;
+            for (int i = 0; i < kDOX; ++i) {
+                const int x = xb + i;
                 if (x < W && y < H) {
-                    float a = tileS[(ty * kDM + j + h) * pitch + tx + h];
-                    dst[y * W + x] = growth_finish(g, a, acc[j], x, y, W, H, seed, ck.frame);
+                    float a = tileS[(ty + h) * pitch + kDOX * tx + i + h];
+                    dst[(size_t)y * W + x] = growth_finish(g, a, gacc[i], x, y, W, H, seed, ck.frame);
                 }
             }
         }
@@ -126,6 +194,14 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
 void direct_tile_shape(int *tx, int *ty) {
     *tx = kDTX;
     *ty = kDTY;
+}
+
+// window row pitch (floats): 2h+1 rounded up to 8
+int direct_npad(int h) { return (2 * h + 1 + 7) & ~7; }
+
+size_t direct_lds_bytes(int h) {
+    const int pitch = kDTX + direct_npad(h) + 4;
+    return (size_t)(kDTY + 2 * h) * pitch * sizeof(float);
 }
 
 }  // namespace lenia

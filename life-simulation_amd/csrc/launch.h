@@ -47,6 +47,8 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
                          int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
                          hipStream_t s);
 void direct_tile_shape(int *tx, int *ty);
+int direct_npad(int h);
+size_t direct_lds_bytes(int h);
 
 // MFMA direct forms (k_mfma.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,

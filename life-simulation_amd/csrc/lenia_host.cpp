@@ -41,6 +41,9 @@ int lenia::hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
+// dynamic LDS a workgroup of this library may request
+static constexpr size_t kLdsMax = 65536;
+
 extern "C" const char *lenia_last_error(void) { return g_err.c_str(); }
 extern "C" const char *lenia_version(void) { return "lenia-mi355x 0.1 (gfx950)"; }
 
@@ -318,10 +321,11 @@ static int check_desc(const lenia_world_desc *d) {
         if (kd.core == LENIA_CORE_GAUSSIAN && !(kd.shellSigma > 0.0f))
             return fail(LENIA_ERR_INVALID_ARG, "kernel %u: shellSigma must be > 0", k);
         int h = kernel_half(kd);
-        if (d->convMode != LENIA_CONV_FFT && h > 45)
+        if (d->convMode == LENIA_CONV_DIRECT && direct_lds_bytes(h) > kLdsMax)
             return fail(LENIA_ERR_UNSUPPORTED,
-                        "kernel %u: direct forms stage a (16+2h)x(64+2h) LDS tile and support "
-                        "h <= 45 (got %d); use the FFT form", k, h);
+                        "kernel %u: the FP32 direct form stages a (16+2h) x (68+P) LDS tile "
+                        "(P = 2h+1 rounded up to 8) and supports h <= 43 (got %d); use the FFT "
+                        "form", k, h);
         if (2 * h + 1 > (int)std::min(d->width, d->height))
             return fail(LENIA_ERR_INVALID_ARG,
                         "kernel %u: window 2*%d+1 exceeds the field (%ux%u); the torus would "
@@ -431,17 +435,23 @@ static int build_kernels_direct(lenia_world *w) {
             return fail(LENIA_ERR_INVALID_ARG, "kernel %u has zero mass", k);
         for (size_t i = 0; i < win.size(); ++i) all[(size_t)k * n * n + i] = (float)win[i];
     }
-    size_t bytes = all.size() * sizeof(float);
+    if (w->isMfma) return build_toeplitz(w, all, h);
+    // FP32 direct form: window rows zero padded to a multiple of 8 taps
+    const int np = direct_npad(h);
+    std::vector<float> padded((size_t)d.nKernels * n * np, 0.0f);
+    for (size_t r = 0; r < (size_t)d.nKernels * n; ++r)
+        for (int x = 0; x < n; ++x) padded[r * np + x] = all[r * n + x];
+    size_t bytes = padded.size() * sizeof(float);
     if (!w->winF || w->winBytes != bytes) {
         dev_free(ctx, w->winF);
         int rc = dev_alloc(ctx, (void **)&w->winF, bytes);
         if (rc) return rc;
         w->winBytes = bytes;
     }
-    HIP_TRY(hipMemcpyAsync(w->winF, all.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
+    HIP_TRY(hipMemcpyAsync(w->winF, padded.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
             "upload kernel windows");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload kernel windows");
-    return w->isMfma ? build_toeplitz(w, all, h) : LENIA_OK;
+    return LENIA_OK;
 }
 
 static int build_kernels_fft(lenia_world *w) {
@@ -484,7 +494,6 @@ static int build_kernels_fft(lenia_world *w) {
 // from the descriptor win.  Otherwise: the smallest tile (nt = 1) with the
 // smallest workgroup that holds <= 16 values per thread, side by side when it
 // fits in 64 KB of LDS.
-static constexpr size_t kLdsMax = 65536;
 
 // The swizzle sw(i) permutes indices inside aligned blocks of 32, so every
 // swizzled region is sized up to a multiple of 32 entries.
@@ -649,6 +658,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         dd.half = w->half;
         dd.tilesX = (dd.W + tx - 1) / tx;
         dd.tilesY = (dd.H + ty - 1) / ty;
+        dd.npad = direct_npad(w->half);
         dd.g = w->g;
         dd.win = w->winF;
         dd.A0 = w->A0;
@@ -662,7 +672,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
                                hipMemcpyHostToDevice, ctx->stream), "upload tiles");
         w->nRowTiles = (int)t.size();
-        w->ldsRows = (size_t)(ty + 2 * w->half) * (tx + 2 * w->half) * sizeof(float);
+        w->ldsRows = direct_lds_bytes(w->half);
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload world descriptor");
     return LENIA_OK;

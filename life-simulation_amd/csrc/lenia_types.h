@@ -89,9 +89,10 @@ struct DirectWorldDev {
     int32_t W, H;
     int32_t half;       // window half-width (max over kernels)
     int32_t tilesX, tilesY;
-    int32_t pad0[3];
+    int32_t npad;       // window row pitch: 2 half + 1 rounded up to 8
+    int32_t pad0[2];
     GrowthParams g;
-    const float *win;   // [K][(2 half+1)^2] float windows (zero padded to half)
+    const float *win;   // [K][2 half + 1][npad] float windows (zero padded)
     float *A0, *A1;     // ping-pong states
     StepClock *clock;
 };
