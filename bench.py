@@ -84,6 +84,7 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
         bytes_step = sum(d["width"] * d["height"] *
                          fft_bytes_per_cell(d["width"], len(d["kernels"]))["step"] for d in descs)
         with lenia.Batch(ctx, worlds) as b:
+            tuned_ms = b.autotune(10)  # launch shapes for this batch (bits unchanged)
             b.step(graph_steps, graph_steps)  # warm-up + graph capture
             ctx.sync()
             b.step(steps, graph_steps)
@@ -106,7 +107,7 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
                 "plain_mcell_steps_per_s": round(plain_rate, 1),
                 "speedup_vs_plain": round(batch_rate / plain_rate, 2),
                 "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                "bytes_per_step": int(bytes_step)}
+                "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5)}
     finally:
         for w in worlds:
             w.close()
@@ -129,6 +130,8 @@ def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
     rng = np.random.default_rng(5)
     gT = rng.standard_normal(A0.shape).astype(np.float32)
     with lenia.World(ctx, dict(desc, initMode=INIT_NONE)) as w:
+        w.write(A0)
+        w.autotune(10)
         w.rollout(A0, T)
         w.rollout_grad(A0, T, gT, ckpt)  # workspace, graphs, spectra
         fwd, bwd = [], []
@@ -265,6 +268,7 @@ def main() -> int:
     with lenia.Context(0) as ctx, lenia.World(ctx, desc) as w:
         w.reset(7777)
         w.frame(0, 1)
+        w.autotune(20)  # launch shapes for this world (bits unchanged)
         w.step(args.warmup)
         ctx.sync()
         t0 = time.perf_counter()
@@ -306,7 +310,8 @@ def main() -> int:
                    "kernels": K, "radius": 24, "growthMode": 0, "dt": 0.1,
                    "fftPlanW": tiles["fftPlanW"], "fftPlanH": tiles["fftPlanH"],
                    "rowPairsPerTile": tiles["rowPairsPerTile"],
-                   "colsPerTile": tiles["colsPerTile"]},
+                   "colsPerTile": tiles["colsPerTile"], "rowThreads": tiles["rowThreads"],
+                   "colThreads": tiles["colThreads"], "autotuned": True},
         "passes_ms": {k: round(v, 5) for k, v in per.items()},
         "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -152,6 +152,14 @@ int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs, uint3
                            uint32_t *nPasses);
 /* resolved descriptor (tile shapes and FFT plans filled in) */
 int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
+/* empirical launch-shape search for the FFT form (no counterpart in the
+ * reference; the PresetLab/LifeBench callers get tuned tiles): every
+ * transforms-per-workgroup, workgroup-size and side-by-side choice the plan
+ * allows is timed over nSteps steps per pass, the fastest kept.  Plans are not
+ * changed, so results are bit-identical before and after; the state and step
+ * clock are restored.  *msPerStep (may be NULL): the tuned kernel time of one
+ * step.  Direct forms: nothing to tune, *msPerStep = 0. */
+int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msPerStep);
 
 /* ---- differentiable rollout (no counterpart in the reference; for callers
  * that fit parameters and initial states to target fields).  FFT form,
@@ -181,6 +189,10 @@ int lenia_batch_destroy(lenia_batch *b);
 /* nSteps steps of every world; graphSteps > 0 captures that many steps in a
  * hipGraph once and replays it (nSteps must then be a multiple of graphSteps) */
 int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
+/* the batch's launch shapes (one workgroup size and side-by-side choice per
+ * pass for all worlds) picked by timing every allowed choice over nSteps steps;
+ * states and clocks restored, results unchanged bit for bit */
+int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep);
 
 #ifdef __cplusplus
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <utility>
 #include <vector>
 
@@ -38,6 +39,7 @@ struct lenia_world {
     hipGraphExec_t graphExec = nullptr;     // captured step sequence
     bool graphExtras = false;
     uint32_t version = 0;  // bumped on every descriptor upload (batches re-read)
+    uint32_t shapeVersion = 0;  // bumped when launch shapes change (captured graphs re-record)
     struct AdjWorkspace *adj = nullptr;  // rollout-gradient buffers, made on first use
     float *F = nullptr, *G = nullptr, *S = nullptr, *twW = nullptr, *twH = nullptr;
     // direct forms
@@ -98,6 +100,36 @@ int world_upload_desc(lenia_world *w);
 int world_launch_steps(lenia_world *w, uint32_t n);
 void world_drop_graph(lenia_world *w);
 void world_free_adjoint(lenia_world *w);
+void world_invalidate_adjoint(lenia_world *w);
+
+// Mean duration in ms of each of the np kernels of a step over nSteps steps:
+// `step(ev)` enqueues one step whose kernel p records ev[2p] and ev[2p+1]
+// around itself (hipExtLaunchKernelGGL timestamps), returning a hipError_t.
+template <class F>
+hipError_t time_steps(lenia_ctx *ctx, uint32_t nSteps, uint32_t np, F step, double *out) {
+    constexpr uint32_t kChunk = 64;  // steps between event read-backs
+    std::vector<hipEvent_t> ev(2 * np * kChunk, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto &x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    std::vector<double> sum(np, 0.0);
+    for (uint32_t done = 0; done < nSteps && e == hipSuccess;) {
+        const uint32_t m = std::min(kChunk, nSteps - done);
+        for (uint32_t i = 0; i < m && e == hipSuccess; ++i) e = step(&ev[2 * np * i]);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        for (uint32_t i = 0; i < m && e == hipSuccess; ++i)
+            for (uint32_t p = 0; p < np && e == hipSuccess; ++p) {
+                float f = 0.f;
+                e = hipEventElapsedTime(&f, ev[2 * np * i + 2 * p], ev[2 * np * i + 2 * p + 1]);
+                sum[p] += f;
+            }
+        done += m;
+    }
+    for (auto x : ev)
+        if (x) (void)hipEventDestroy(x);
+    for (uint32_t p = 0; p < np; ++p) out[p] = sum[p] / std::max(1u, nSteps);
+    return e;
+}
 
 uint32_t pcg_hash_host(uint32_t v);
 uint32_t module_seed(uint32_t sceneSeed);

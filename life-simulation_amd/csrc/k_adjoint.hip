@@ -15,9 +15,21 @@
 // In the spectral domain the adjoint convolution is the conjugate-spectrum
 // product: multiply by the (real, even) kernel spectrum and transform back.
 //
-// These kernels reuse the LDS FFT of the forward path (same plans, same
-// butterflies).  Partial sums are written per workgroup and summed in a fixed
-// order, so gradients are identical run to run.
+// One reverse step is four launches (row transforms are along x on row pairs
+// packed as one complex row, column transforms along y on the half spectrum):
+//   k_adj_cols<POT>  F_t (row spectrum of A_t) -> Ahat (2D spectrum),
+//                    G_k = column inverse of S_k Ahat
+//   k_adj_mid        row inverse of G_k -> u_k; per cell gp, gu_k and the
+//                    parameter partials; row forward of gu_k -> FG_k
+//   k_adj_cols<ADJ>  column forward of FG_k; shell partials against Ahat;
+//                    G1 = column inverse of sum_k S_k FG_k^
+//   k_adj_edge       g_t = gp + row inverse of G1, and the row spectrum
+//                    F_{t-1} of the previous state for the next reverse step
+// The kernels reuse the LDS FFT of the forward path (same plans, same
+// butterflies).  Each workgroup adds its partial sums to its own FP64 slots
+// (one row pair or one spectrum column per workgroup, every step in order),
+// and a final kernel sums the slots in a fixed order, so gradients are
+// identical run to run.
 #include <hip/hip_runtime.h>
 
 #include "fft_device.h"
@@ -28,89 +40,192 @@ namespace lenia {
 
 constexpr int kAV = 16;  // values per thread per FFT stage
 
-// ------------------------------------------------------------------ rows: forward R2C
-// nf real fields (field f at in + f*fieldStride) -> half spectra
-// out[f][kx][y] (complex, column-major), one row pair per workgroup
-template <int T>
-__global__ __launch_bounds__(T) void k_adj_rows_fwd(AdjArgs a, const float *in, size_t fieldStride,
-                                                    float *out, int nf) {
-    extern __shared__ float2 smem[];
-    const int tid = threadIdx.x;
-    const int W = a.W, H = a.H, Wh = W / 2 + 1, H2 = H / 2;
-    const int rp = blockIdx.x;  // row pair
-    float2 *tw = smem;
-    float2 *Z = smem + W;
-    for (int t = tid; t < W; t += T) tw[t] = gld2(gp(reinterpret_cast<const float2 *>(a.twW)) + t);
-    for (int q = tid; q < nf * W; q += T) {
-        int f = q / W, x = q - f * W;
-        const LENIA_GLOBAL float *src = gp(in) + f * fieldStride + (size_t)(2 * rp) * W;
-        Z[sw(q)] = make_float2(src[x], src[x + W]);
-    }
-    __syncthreads();
-    fft_lds<false, kAV, T>(Z, a.planW, nf, tw, tid);
-    for (int m = tid; m < nf * Wh; m += T) {
-        int f = m / Wh, kx = m - f * Wh;
-        float2 z1 = Z[sw(f * W + kx)];
-        float2 z2 = Z[sw(f * W + (kx == 0 ? 0 : W - kx))];
-        LENIA_GLOBAL float4 *o = gp(reinterpret_cast<float4 *>(out)) + (size_t)f * Wh * H2;
-        gst4(o + kx * H2 + rp, make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
-                                           0.5f * (z1.y + z2.y), 0.5f * (z2.x - z1.x)));
-    }
+__device__ __forceinline__ void load_twiddles(float2 *tw, const float *src, int n, int tid, int T) {
+    for (int t = tid; t < n; t += T) tw[t] = gld2(gp(reinterpret_cast<const float2 *>(src)) + t);
 }
 
-// ------------------------------------------------------------------ rows: inverse C2R
-// ns half-spectrum slots G[s][kx][y] -> real fields out[s][y][x]; with addend,
-// out = addend + field (one slot)
-template <int T>
-__global__ __launch_bounds__(T) void k_adj_rows_inv(AdjArgs a, const float *G, float *out, int ns,
-                                                    const float *addend) {
-    extern __shared__ float2 smem[];
-    const int tid = threadIdx.x;
-    const int W = a.W, H = a.H, Wh = W / 2 + 1, H2 = H / 2;
-    const int rp = blockIdx.x;
-    float2 *tw = smem;
-    float2 *Z = smem + W;
-    for (int t = tid; t < W; t += T) tw[t] = gld2(gp(reinterpret_cast<const float2 *>(a.twW)) + t);
-    for (int m = tid; m < ns * Wh; m += T) {
-        int s = m / Wh, kx = m - s * Wh;
-        float4 g4 = gld4(gp(reinterpret_cast<const float4 *>(G)) + (size_t)s * Wh * H2 + kx * H2 + rp);
+// half spectrum slot (float4 per kx: rows 2rp and 2rp+1) -> packed complex row
+// Z[base + x] = a[x] + i b[x] (Hermitian mirror fills the negative
+// frequencies; conjugated when CONJ).  LDS indices are swizzled as a whole:
+// slot s of an FFT batch lives at sw(s W + x), not at s W + sw(x).
+template <bool CONJ>
+__device__ __forceinline__ void gather_row_pair(float2 *Z, int base, const float *G, int W, int H2,
+                                                int rp, int tid, int T) {
+    const int Wh = W / 2 + 1;
+    const float sg = CONJ ? -1.0f : 1.0f;
+    for (int kx = tid; kx < Wh; kx += T) {
+        float4 g4 = gld4(gp(reinterpret_cast<const float4 *>(G)) + (size_t)kx * H2 + rp);
         if (kx == 0 || kx == W / 2) {
             g4.y = 0.0f;
             g4.w = 0.0f;
         }
-        Z[sw(s * W + kx)] = make_float2(g4.x - g4.w, g4.y + g4.z);
-        if (kx > 0 && kx < W / 2) Z[sw(s * W + W - kx)] = make_float2(g4.x + g4.w, g4.z - g4.y);
+        Z[sw(base + kx)] = make_float2(g4.x - g4.w, sg * (g4.y + g4.z));
+        if (kx > 0 && kx < W / 2) Z[sw(base + W - kx)] = make_float2(g4.x + g4.w, sg * (g4.z - g4.y));
+    }
+}
+
+// packed complex row spectrum at Z[base ..] (two real rows a + i b) -> their
+// half spectra, stored as one float4 per kx
+__device__ __forceinline__ void split_row_pair(const float2 *Z, int base, float *out, int W, int H2,
+                                               int rp, int tid, int T) {
+    const int Wh = W / 2 + 1;
+    LENIA_GLOBAL float4 *o = gp(reinterpret_cast<float4 *>(out));
+    for (int kx = tid; kx < Wh; kx += T) {
+        float2 z1 = Z[sw(base + kx)];
+        float2 z2 = Z[sw(base + (kx == 0 ? 0 : W - kx))];
+        gst4(o + (size_t)kx * H2 + rp, make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
+                                                   0.5f * (z1.y + z2.y), 0.5f * (z2.x - z1.x)));
+    }
+}
+
+// ------------------------------------------------------------------ rows: edge
+// slot 0: g = gpb + row inverse of G1 (if G1); slot 1: row spectrum of the
+// state Anext (if Anext).  The inverse runs as a forward transform of the
+// conjugate, so both slots share one pass.  One row pair per workgroup.
+template <int T>
+__global__ __launch_bounds__(T) void k_adj_edge(AdjArgs a, const float *G1, const float *gpb,
+                                                float *gOut, const float *Anext, float *Fnext) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const int W = a.W, H2 = a.H / 2;
+    const int rp = blockIdx.x;
+    float2 *tw = smem;
+    float2 *Z = smem + W;
+    load_twiddles(tw, a.twW, W, tid, T);
+    // slot 0 holds the conjugate: inverse = conj(forward(conj))
+    if (G1) {
+        gather_row_pair<true>(Z, 0, G1, W, H2, rp, tid, T);
+    } else {
+        for (int x = tid; x < W; x += T) Z[sw(x)] = make_float2(0.f, 0.f);
+    }
+    const LENIA_GLOBAL float *an = gp(Anext);
+    for (int x = tid; x < W; x += T)
+        Z[sw(W + x)] = Anext ? make_float2(an[(size_t)(2 * rp) * W + x], an[(size_t)(2 * rp + 1) * W + x])
+                             : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft_lds<false, kAV, T>(Z, a.planW, 2, tw, tid);
+    if (G1) {
+        const LENIA_GLOBAL float *ad = gp(gpb);
+        LENIA_GLOBAL float *go = gp(gOut);
+        for (int x = tid; x < W; x += T) {
+            float2 z = Z[sw(x)];  // conj of the inverse: real parts unchanged, imag negated
+            const size_t i0 = (size_t)(2 * rp) * W + x;
+            go[i0] = ad[i0] + z.x;
+            go[i0 + W] = ad[i0 + W] - z.y;
+        }
+    }
+    if (Anext) split_row_pair(Z, W, Fnext, W, H2, rp, tid, T);
+}
+
+// ------------------------------------------------------------------ rows: middle
+// row inverse of the K slots of Gk -> u_k rows; cells -> gp (gpb), gu_k;
+// row forward of gu_k -> FG.  Partials of dt, mu_k, sigma_k, w_k are added to
+// accRows[rp][v] (FP64).
+template <int T>
+__global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *Gk, const float *A,
+                                               const float *g, float *gpb, float *FG,
+                                               double *accRows) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const int W = a.W, H = a.H, Wh = W / 2 + 1, H2 = H / 2, K = a.K;
+    const int rp = blockIdx.x;
+    const size_t sStride = (size_t)Wh * H;
+    float2 *tw = smem;
+    float2 *Z = smem + W;
+    float *red = reinterpret_cast<float *>(Z + ((K * W + 31) & ~31));
+    load_twiddles(tw, a.twW, W, tid, T);
+    for (int k = 0; k < K; ++k)
+        gather_row_pair<false>(Z, k * W, Gk + k * sStride * 2, W, H2, rp, tid, T);
+    __syncthreads();
+    fft_lds<true, kAV, T>(Z, a.planW, K, tw, tid);
+    const GrowthParams &gr = *a.gDev;
+    const float dt = gr.dt;
+    float mu[4], bs[4], wk[4], sg[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        mu[k] = gr.mu[k];
+        bs[k] = gr.bellScale[k];
+        wk[k] = gr.weight[k];
+        sg[k] = gr.sigma[k];
+    }
+    float vals[13];
+#pragma unroll
+    for (int v = 0; v < 13; ++v) vals[v] = 0.0f;
+    const LENIA_GLOBAL float *Ag = gp(A);
+    const LENIA_GLOBAL float *gg = gp(g);
+    LENIA_GLOBAL float *gpo = gp(gpb);
+    for (int x = tid; x < W; x += T) {
+        float2 u2[4], gu2[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            u2[k] = k < K ? Z[sw(k * W + x)] : make_float2(0.f, 0.f);
+            gu2[k] = make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const size_t i = (size_t)(2 * rp + half) * W + x;
+            float u[4], b[4], sumStd = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u[k] = half ? u2[k].y : u2[k].x;
+                b[k] = bell_of(u[k], mu[k], bs[k]);
+                if (k < K) sumStd += wk[k] * (2.0f * b[k] - 1.0f);
+            }
+            const float pre = Ag[i] + dt * sumStd;
+            const float gpv = (pre > 0.0f && pre < 1.0f) ? gg[i] : 0.0f;
+            gpo[i] = gpv;
+            vals[0] += gpv * sumStd;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < K) {
+                    const float d = u[k] - mu[k];
+                    const float cterm = gpv * dt * wk[k] * 2.0f * b[k];
+                    const float is2 = 1.0f / (sg[k] * sg[k]);
+                    vals[1 + 3 * k] += cterm * d * is2;
+                    vals[2 + 3 * k] += cterm * d * d * is2 / sg[k];
+                    vals[3 + 3 * k] += gpv * dt * (2.0f * b[k] - 1.0f);
+                    const float gu = -cterm * d * is2;
+                    if (half) gu2[k].y = gu; else gu2[k].x = gu;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < K) Z[sw(k * W + x)] = gu2[k];
     }
     __syncthreads();
-    fft_lds<true, kAV, T>(Z, a.planW, ns, tw, tid);
-    for (int q = tid; q < ns * W; q += T) {
-        int s = q / W, x = q - s * W;
-        float2 z = Z[sw(q)];
-        size_t ia = (size_t)s * W * H + (size_t)(2 * rp) * W + x;
-        if (addend) {
-            z.x += gp(addend)[ia];
-            z.y += gp(addend)[ia + W];
+    fft_lds<false, kAV, T>(Z, a.planW, K, tw, tid);
+    for (int k = 0; k < K; ++k) split_row_pair(Z, k * W, FG + k * sStride * 2, W, H2, rp, tid, T);
+    // fixed-order workgroup sums of the partials
+    const int nv = 1 + 3 * K;
+    for (int v = 0; v < nv; ++v) {
+        red[tid] = vals[v];
+        __syncthreads();
+        for (int s = T / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] += red[tid + s];
+            __syncthreads();
         }
-        gp(out)[ia] = z.x;
-        gp(out)[ia + W] = z.y;
+        if (tid == 0) gp(accRows)[(size_t)rp * 16 + v] += (double)red[0];
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ columns
 // POT: F -> (store the forward column spectrum Ahat) -> x S_k -> inverse -> G[k]
 // ADJ: FG[k] (k < K) -> forward -> spectral partial sums for the shell
-//      gradients against Ahat -> Y = sum_k S_k FG^[k] -> inverse -> G[0]
+//      gradients against Ahat (added to accCols[c][i]) -> Y = sum_k S_k FG^[k]
+//      -> inverse -> G[0]
 template <int T, bool ADJ>
 __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, float *Ahat,
-                                                float *G, float *betaPart) {
+                                                float *G, double *accCols) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
     const int H = a.H, Wh = a.W / 2 + 1, K = a.K;
     const int c = blockIdx.x;  // spectrum column kx
     const size_t sStride = (size_t)Wh * H;
     float2 *tw = smem;
-    float2 *X = smem + H;  // ADJ: K slots; POT: 1 slot + K slots
-    for (int t = tid; t < H; t += T) tw[t] = gld2(gp(reinterpret_cast<const float2 *>(a.twH)) + t);
+    float2 *X = smem + H;  // ADJ: K slots; POT: 1 slot
+    load_twiddles(tw, a.twH, H, tid, T);
     const int nin = ADJ ? K : 1;
     for (int e = tid; e < nin * H; e += T) {
         int s = e / H, ky = e - s * H;
@@ -142,11 +257,14 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, flo
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
     for (int ky = tid; ky < H; ky += T) {
         float2 ah = gld2(gp(reinterpret_cast<const float2 *>(Ahat)) + (size_t)c * H + ky);
-        for (int i = 0; i < nb; ++i) {
-            int k = a.shellK[i];
-            float2 gu = X[sw(k * H + ky)];
-            float re = gu.x * ah.x + gu.y * ah.y;  // Re(conj(gu) * A)
-            acc[i] += gp(a.D)[(size_t)i * sStride + (size_t)c * H + ky] * re;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            if (i < nb) {
+                int k = a.shellK[i];
+                float2 gu = X[sw(k * H + ky)];
+                float re = gu.x * ah.x + gu.y * ah.y;  // Re(conj(gu) * A)
+                acc[i] += gp(a.D)[(size_t)i * sStride + (size_t)c * H + ky] * re;
+            }
         }
     }
     // Y = sum_k S_k * gu_k^
@@ -160,7 +278,7 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, flo
         }
         Y[sw(e)] = y;
     }
-    // reduce the partial sums across the workgroup (reuse the tail of LDS)
+    // reduce the partial sums across the workgroup (the LDS after Y)
     float *red = reinterpret_cast<float *>(Y + ((H + 31) & ~31));
     for (int i = 0; i < nb; ++i) {
         red[tid] = acc[i] * wt;
@@ -169,78 +287,33 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, flo
             if (tid < s) red[tid] += red[tid + s];
             __syncthreads();
         }
-        if (tid == 0) gp(betaPart)[(size_t)c * 16 + i] = red[0];
+        if (tid == 0) gp(accCols)[(size_t)c * 16 + i] += (double)red[0];
         __syncthreads();
     }
     fft_lds<true, kAV, T>(Y, a.planH, 1, tw, tid);
     for (int e = tid; e < H; e += T) gst2(gp(reinterpret_cast<float2 *>(G)) + (size_t)c * H + e, Y[sw(e)]);
 }
 
-// ------------------------------------------------------------------ cells
-// per cell: gp, gu_k and the cell terms of the parameter gradients; block
-// partial sums in fixed tree order: part[block][0] = dt, then per k: mu,
-// sigma, weight at 1 + 3k + {0,1,2}
-__global__ __launch_bounds__(256) void k_adj_cells(AdjArgs a, const float *A, const float *U,
-                                                   const float *g, float *gpOut, float *GU,
-                                                   float *part) {
-    __shared__ float red[256];
-    const int tid = threadIdx.x;
-    const size_t n = (size_t)a.W * a.H;
-    const size_t i = (size_t)blockIdx.x * 256 + tid;
-    float vals[13];
-#pragma unroll
-    for (int v = 0; v < 13; ++v) vals[v] = 0.0f;
-    if (i < n) {
-        float A0 = gp(A)[i];
-        float b[4], u[4], sumStd = 0.0f;
-        for (int k = 0; k < a.K; ++k) {
-            u[k] = gp(U)[k * n + i];
-            b[k] = bell_of(u[k], a.g.mu[k], a.g.bellScale[k]);
-            sumStd += a.g.weight[k] * (2.0f * b[k] - 1.0f);
-        }
-        float pre = A0 + a.g.dt * sumStd;
-        float gin = gp(g)[i];
-        float gpv = (pre > 0.0f && pre < 1.0f) ? gin : 0.0f;
-        gp(gpOut)[i] = gpv;
-        vals[0] = gpv * sumStd;
-        for (int k = 0; k < a.K; ++k) {
-            float sg = a.g.sigma[k], d = u[k] - a.g.mu[k];
-            float cterm = gpv * a.g.dt * a.g.weight[k] * 2.0f * b[k];
-            float is2 = 1.0f / (sg * sg);
-            vals[1 + 3 * k] = cterm * d * is2;
-            vals[2 + 3 * k] = cterm * d * d * is2 / sg;
-            vals[3 + 3 * k] = gpv * a.g.dt * (2.0f * b[k] - 1.0f);
-            gp(GU)[k * n + i] = -cterm * d * is2;
-        }
-    }
-    const int nv = 1 + 3 * a.K;
-    for (int v = 0; v < nv; ++v) {
-        red[tid] = vals[v];
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) gp(part)[(size_t)blockIdx.x * 16 + v] = red[0];
-        __syncthreads();
-    }
-}
-
-// fixed-order sums of block partials, accumulated in FP64:
-// acc[v] += sum_b part[b][v] for v < nv
-__global__ void k_adj_reduce(const float *part, int nBlocks, int nv, double *acc) {
+// ------------------------------------------------------------------ final sums
+// acc[v] = sum_rp accRows[rp][v] (v < nvRows), acc[16 + i] = sum_c accCols[c][i]
+// (i < nvCols), each a fixed-order FP64 tree
+__global__ void k_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,
+                            int nCols, int nvCols, double *acc) {
     __shared__ double red[256];
     const int tid = threadIdx.x;
-    for (int v = 0; v < nv; ++v) {
+    for (int v = 0; v < nvRows + nvCols; ++v) {
+        const bool rows = v < nvRows;
+        const double *src = rows ? accRows : accCols;
+        const int n = rows ? nRows : nCols, col = rows ? v : v - nvRows;
         double s = 0.0;
-        for (int b = tid; b < nBlocks; b += 256) s += (double)gp(part)[(size_t)b * 16 + v];
+        for (int b = tid; b < n; b += 256) s += gp(src)[(size_t)b * 16 + col];
         red[tid] = s;
         __syncthreads();
         for (int k = 128; k > 0; k >>= 1) {
             if (tid < k) red[tid] += red[tid + k];
             __syncthreads();
         }
-        if (tid == 0) gp(acc)[v] += red[0];
+        if (tid == 0) gp(acc)[rows ? v : 16 + col] = red[0];
         __syncthreads();
     }
 }
@@ -258,58 +331,64 @@ static hipError_t by_threads(int T, F f) {
     return hipGetLastError();
 }
 
-// LDS: the twiddles, then the swizzled region rounded up to 32 entries (sw()
+// LDS: the twiddles, then the swizzled regions rounded up to 32 entries (sw()
 // permutes inside aligned blocks of 32)
 static inline int r32(int n) { return (n + 31) & ~31; }
 
-hipError_t launch_adj_rows_fwd(const AdjArgs &a, int T, const float *in, size_t fieldStride,
-                               float *out, int nf, hipStream_t s) {
-    size_t lds = (size_t)(a.W + r32(nf * a.W)) * sizeof(float2);
+size_t adj_lds_rows(const AdjArgs &a, int T) {
+    const size_t edge = (size_t)(a.W + r32(2 * a.W)) * sizeof(float2);
+    const size_t mid = (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
+    return edge > mid ? edge : mid;
+}
+
+size_t adj_lds_cols(const AdjArgs &a, int T) {
+    // POT: twiddles | X [1][H] | Y [K][H];  ADJ: twiddles | X [K][H] | Y [H] | reduction [T]
+    const size_t pot = (size_t)(a.H + r32(a.H) + r32(a.K * a.H)) * sizeof(float2);
+    const size_t adj = (size_t)(a.H + r32(a.K * a.H) + r32(a.H)) * sizeof(float2) + (size_t)T * sizeof(float);
+    return pot > adj ? pot : adj;
+}
+
+hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
+                           const float *Anext, float *Fnext, hipStream_t s) {
+    const size_t lds = (size_t)(a.W + r32(2 * a.W)) * sizeof(float2);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        hipLaunchKernelGGL(k_adj_rows_fwd<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, in, fieldStride,
-                           out, nf);
+        hipLaunchKernelGGL(k_adj_edge<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, G1, gpb, gOut, Anext,
+                           Fnext);
     });
 }
 
-hipError_t launch_adj_rows_inv(const AdjArgs &a, int T, const float *G, float *out, int ns,
-                               const float *addend, hipStream_t s) {
-    size_t lds = (size_t)(a.W + r32(ns * a.W)) * sizeof(float2);
+hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *Gk, const float *A, const float *g,
+                          float *gpb, float *FG, double *accRows, hipStream_t s) {
+    const size_t lds = (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        hipLaunchKernelGGL(k_adj_rows_inv<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, G, out, ns,
-                           addend);
+        hipLaunchKernelGGL(k_adj_mid<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, Gk, A, g, gpb, FG,
+                           accRows);
     });
 }
 
 hipError_t launch_adj_cols(const AdjArgs &a, int T, bool adj, const float *Fin, float *Ahat,
-                           float *G, float *betaPart, hipStream_t s) {
+                           float *G, double *accCols, hipStream_t s) {
     const int nin = adj ? a.K : 1;
-    // twiddles | X [nin][H] | POT: Y [K][H] ; ADJ: Y [H] | reduction [T <= 512]
     size_t lds = (size_t)(a.H + r32(nin * a.H) + (adj ? r32(a.H) : r32(a.K * a.H))) * sizeof(float2) +
-                 (adj ? 512 * sizeof(float) : 0);
+                 (adj ? (size_t)T * sizeof(float) : 0);
     const int Wh = a.W / 2 + 1;
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
         if (adj)
             hipLaunchKernelGGL((k_adj_cols<TT, true>), dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat, G,
-                               betaPart);
+                               accCols);
         else
-            hipLaunchKernelGGL((k_adj_cols<TT, false>), dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat,
-                               G, betaPart);
+            hipLaunchKernelGGL((k_adj_cols<TT, false>), dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat, G,
+                               accCols);
     });
 }
 
-hipError_t launch_adj_cells(const AdjArgs &a, const float *A, const float *U, const float *g,
-                            float *gpOut, float *GU, float *part, hipStream_t s) {
-    const size_t n = (size_t)a.W * a.H;
-    hipLaunchKernelGGL(k_adj_cells, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, A, U,
-                       g, gpOut, GU, part);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_reduce(const float *part, int nBlocks, int nv, double *acc, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(256), 0, s, part, nBlocks, nv, acc);
+hipError_t launch_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,
+                            int nCols, int nvCols, double *acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(256), 0, s, accRows, nRows, nvRows, accCols, nCols,
+                       nvCols, acc);
     return hipGetLastError();
 }
 

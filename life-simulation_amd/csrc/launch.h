@@ -59,15 +59,16 @@ int mfma_max_ksteps();
 size_t mfma_lds_bytes(int h, bool split);
 
 // adjoint (k_adjoint.hip)
-hipError_t launch_adj_rows_fwd(const AdjArgs &a, int T, const float *in, size_t fieldStride,
-                               float *out, int nf, hipStream_t s);
-hipError_t launch_adj_rows_inv(const AdjArgs &a, int T, const float *G, float *out, int ns,
-                               const float *addend, hipStream_t s);
+hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
+                           const float *Anext, float *Fnext, hipStream_t s);
+hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *Gk, const float *A, const float *g,
+                          float *gpb, float *FG, double *accRows, hipStream_t s);
 hipError_t launch_adj_cols(const AdjArgs &a, int T, bool adj, const float *Fin, float *Ahat,
-                           float *G, float *betaPart, hipStream_t s);
-hipError_t launch_adj_cells(const AdjArgs &a, const float *A, const float *U, const float *g,
-                            float *gpOut, float *GU, float *part, hipStream_t s);
-hipError_t launch_adj_reduce(const float *part, int nBlocks, int nv, double *acc, hipStream_t s);
+                           float *G, double *accCols, hipStream_t s);
+hipError_t launch_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,
+                            int nCols, int nvCols, double *acc, hipStream_t s);
+size_t adj_lds_rows(const AdjArgs &a, int T);
+size_t adj_lds_cols(const AdjArgs &a, int T);
 
 // init and clock (k_init.hip)
 hipError_t launch_init_patch(float *A, int W, int H, uint32_t seed, int x0, int y0, int pw,

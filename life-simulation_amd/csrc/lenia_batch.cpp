@@ -35,6 +35,9 @@ struct lenia_batch {
     hipGraphExec_t graph = nullptr;
     uint32_t graphSteps = 0;
     std::vector<uint32_t> versions;  // world descriptor versions the table holds
+    // launch shapes fixed by lenia_batch_autotune (0: the default choice)
+    int tuneRowT = 0, tuneColT = 0;
+    bool tuneRowKpar = true, tuneColKpar = true;
 };
 
 static void batch_release_tables(lenia_batch *b) {
@@ -79,11 +82,12 @@ static int batch_setup_fft(lenia_batch *b) {
     lenia_ctx *ctx = b->ctx;
     const int n = (int)b->worlds.size();
     // one workgroup size per pass for the whole batch; side-by-side kernels
-    // when every world allows it
+    // when every world allows it (or the tuned shapes)
     const int Ts[4] = {256, 512, 128, 64};
     bool found = false;
     for (int kp = 1; kp >= 0 && !found; --kp)
         for (int T : Ts) {
+            if (b->tuneRowT && (T != b->tuneRowT || (kp != 0) != b->tuneRowKpar)) continue;
             bool ok = true;
             for (lenia_world *w : b->worlds) {
                 const int K = (int)w->desc.nKernels;
@@ -100,6 +104,7 @@ static int batch_setup_fft(lenia_batch *b) {
     found = false;
     for (int kp = 1; kp >= 0 && !found; --kp)
         for (int T : Ts) {
+            if (b->tuneColT && (T != b->tuneColT || (kp != 0) != b->tuneColKpar)) continue;
             bool ok = true;
             for (lenia_world *w : b->worlds) {
                 const int K = (int)w->desc.nKernels;
@@ -223,29 +228,32 @@ extern "C" int lenia_batch_destroy(lenia_batch *b) {
     return LENIA_OK;
 }
 
-static hipError_t batch_enqueue_step(lenia_batch *b) {
+// one step of every world; with `ev`, kernel p records ev[2p] and ev[2p+1]
+static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev = nullptr) {
     lenia_ctx *ctx = b->ctx;
     const int n = (int)b->worlds.size();
+    hipError_t e;
     if (b->isFft) {
-        hipError_t e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds,
-                                       b->colTiles, b->nColTiles, b->nColTiles, b->ldsCols,
-                                       ctx->stream);
-        if (e != hipSuccess) return e;
-        return launch_fft_rows(1, b->rowT, b->extras, b->rowKpar,
-                               (const FftWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                               b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
+        if (ev) launch_events() = {ev[0], ev[1]};
+        e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds, b->colTiles,
+                            b->nColTiles, b->nColTiles, b->ldsCols, ctx->stream);
+        if (ev) launch_events() = {ev[2], ev[3]};
+        if (e == hipSuccess)
+            e = launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, (const FftWorldDev *)b->devWorlds,
+                                b->rowTiles, b->nRowTiles, b->nRowTiles, b->ldsRows, b->done, n, 0,
+                                nullptr, ctx->stream);
+    } else {
+        if (ev) launch_events() = {ev[0], ev[1]};
+        e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
+                          b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
     }
-    return launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                         b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
+    launch_events() = LaunchEvents();
+    return e;
 }
 
-extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
-    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: NULL batch");
-    if (graphSteps && nSteps % graphSteps)
-        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: nSteps %u is not a multiple of %u",
-                    nSteps, graphSteps);
+// table up to date, every world initialised and holding its row spectrum
+static int batch_prepare(lenia_batch *b) {
     lenia_ctx *ctx = b->ctx;
-    (void)hipSetDevice(ctx->device);
     bool stale = false;
     for (size_t i = 0; i < b->worlds.size(); ++i) stale = stale || b->versions[i] != b->worlds[i]->version;
     if (stale) {
@@ -266,6 +274,122 @@ extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphS
         if (!rc) rc = world_ensure_spectrum(w);
         if (rc) return rc;
     }
+    return LENIA_OK;
+}
+
+// Empirical launch shapes for the whole batch (FFT form): each pass's
+// workgroup size and side-by-side choice are timed over nSteps steps with
+// per-kernel timestamps; the worlds' states and clocks are restored.  Plans
+// do not change, so results stay bit-identical to stepping each world alone.
+extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_autotune: NULL batch");
+    if (!b->isFft) {
+        if (msPerStep) *msPerStep = 0.0;
+        return LENIA_OK;
+    }
+    lenia_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    nSteps = std::max(nSteps, 4u);
+    int rc = batch_prepare(b);
+    if (rc) return rc;
+    // save states and clocks
+    size_t total = 0;
+    for (lenia_world *w : b->worlds) total += (size_t)w->desc.width * w->desc.height;
+    float *saved = nullptr;
+    rc = dev_alloc(ctx, (void **)&saved, total * sizeof(float));
+    if (rc) return rc;
+    std::vector<StepClock> clocks(b->worlds.size());
+    hipError_t e = hipSuccess;
+    size_t off = 0;
+    for (size_t i = 0; i < b->worlds.size() && e == hipSuccess; ++i) {
+        lenia_world *w = b->worlds[i];
+        const size_t cells = (size_t)w->desc.width * w->desc.height;
+        e = hipMemcpyAsync(saved + off, w->current_state(), cells * sizeof(float),
+                           hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&clocks[i], w->clock, sizeof(StepClock), hipMemcpyDeviceToHost, ctx->stream);
+        off += cells;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "batch autotune save");
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    auto time_now = [&](double ms[2]) -> int {
+        int r = batch_setup(b);
+        if (r) return r;
+        hipError_t et = hipSuccess;
+        for (int i = 0; i < 2 && et == hipSuccess; ++i) et = batch_enqueue_step(b);
+        if (et == hipSuccess)
+            et = time_steps(ctx, nSteps, 2, [&](const hipEvent_t *ev) { return batch_enqueue_step(b, ev); }, ms);
+        return et == hipSuccess ? LENIA_OK : hip_fail(et, "batch autotune timing");
+    };
+    double best[2] = {1e30, 1e30};
+    int bestRowT = 0, bestColT = 0;
+    bool bestRowKp = true, bestColKp = true;
+    // columns keep their default while the row shapes are timed, and back
+    b->tuneColT = 0;
+    for (int kp = 1; kp >= 0 && !rc; --kp)
+        for (int T = 64; T <= 512 && !rc; T *= 2) {
+            b->tuneRowT = T;
+            b->tuneRowKpar = kp != 0;
+            double ms[2];
+            if (time_now(ms) != LENIA_OK) continue;  // shape not valid for every world
+            if (ms[1] < best[1]) {
+                best[1] = ms[1];
+                bestRowT = T;
+                bestRowKp = kp != 0;
+            }
+        }
+    b->tuneRowT = bestRowT;
+    b->tuneRowKpar = bestRowKp;
+    for (int kp = 1; kp >= 0 && !rc && bestRowT; --kp)
+        for (int T = 64; T <= 512; T *= 2) {
+            b->tuneColT = T;
+            b->tuneColKpar = kp != 0;
+            double ms[2];
+            if (time_now(ms) != LENIA_OK) continue;
+            if (ms[0] < best[0]) {
+                best[0] = ms[0];
+                bestColT = T;
+                bestColKp = kp != 0;
+            }
+        }
+    b->tuneColT = bestColT;
+    b->tuneColKpar = bestColKp;
+    const int rc2 = batch_setup(b);
+    // restore
+    off = 0;
+    e = hipSuccess;
+    for (size_t i = 0; i < b->worlds.size() && e == hipSuccess; ++i) {
+        lenia_world *w = b->worlds[i];
+        const size_t cells = (size_t)w->desc.width * w->desc.height;
+        e = hipMemcpyAsync(w->current_state(), saved + off, cells * sizeof(float),
+                           hipMemcpyDeviceToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_set_clock(w->clock, clocks[i].frame, clocks[i].sub, clocks[i].substeps,
+                                 clocks[i].parity, ctx->stream);
+        w->specValid = false;
+        off += cells;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, saved);
+    if (rc) return rc;
+    if (!bestRowT || !bestColT) return fail(LENIA_ERR_UNSUPPORTED, "batch autotune: no valid shape");
+    if (rc2) return rc2;
+    if (e != hipSuccess) return hip_fail(e, "batch autotune restore");
+    if (msPerStep) *msPerStep = best[0] + best[1];
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: NULL batch");
+    if (graphSteps && nSteps % graphSteps)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: nSteps %u is not a multiple of %u",
+                    nSteps, graphSteps);
+    lenia_ctx *ctx = b->ctx;
+    (void)hipSetDevice(ctx->device);
+    int rc = batch_prepare(b);
+    if (rc) return rc;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     if (graphSteps) {
         if (!b->graph || b->graphSteps != graphSteps) {

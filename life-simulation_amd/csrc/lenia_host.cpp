@@ -1020,7 +1020,7 @@ extern "C" int lenia_world_set_params(lenia_world *w, const lenia_world_desc *de
     w->desc.colThreads = keepCT;
     w->g = growth_of(desc, w->moduleSeed);
     world_drop_graph(w);
-    if (shapes) world_free_adjoint(w);
+    if (shapes) world_invalidate_adjoint(w);
     if (shapes) {
         rc = w->isFft ? build_kernels_fft(w) : build_kernels_direct(w);
         if (rc) return rc;
@@ -1137,31 +1137,13 @@ extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *o
     }
     int rc = world_ensure_spectrum(w);
     if (rc) return rc;
-    constexpr uint32_t kChunk = 64;  // steps between event read-backs
-    std::vector<hipEvent_t> ev(4 * kChunk, nullptr);
-    hipError_t e = hipSuccess;
-    for (auto &x : ev)
-        if (e == hipSuccess) e = hipEventCreate(&x);
-    double sum[2] = {0.0, 0.0};
-    for (uint32_t done = 0; done < nSteps && e == hipSuccess;) {
-        const uint32_t m = std::min(kChunk, nSteps - done);
-        uint32_t launched = 0;
-        for (; launched < m && e == hipSuccess; ++launched) e = enqueue_step(w, &ev[4 * launched]);
-        w->hostSteps += launched;
-        w->frameNext += launched;
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        for (uint32_t i = 0; i < m && e == hipSuccess; ++i)
-            for (uint32_t p = 0; p < np && e == hipSuccess; ++p) {
-                float f = 0.f;
-                e = hipEventElapsedTime(&f, ev[4 * i + 2 * p], ev[4 * i + 2 * p + 1]);
-                sum[p] += f;
-            }
-        done += m;
-    }
-    for (auto x : ev)
-        if (x) (void)hipEventDestroy(x);
+    hipError_t e = time_steps(ctx, nSteps, np, [&](const hipEvent_t *ev) {
+        hipError_t es = enqueue_step(w, ev);
+        w->hostSteps += 1;
+        w->frameNext += 1;
+        return es;
+    }, outMs);
     if (e != hipSuccess) return hip_fail(e, "pass timing");
-    for (uint32_t p = 0; p < np; ++p) outMs[p] = sum[p] / nSteps;
     *nPasses = np;
     return LENIA_OK;
 }

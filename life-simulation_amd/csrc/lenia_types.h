@@ -117,7 +117,7 @@ struct AdjArgs {
     int32_t W, H, K, nShells;
     int32_t shellK[16];  // kernel of each (k, j) shell pair
     FftPlan planW, planH;
-    GrowthParams g;
+    const GrowthParams *gDev;  // the world's growth parameters (device copy)
     const float *twW, *twH;  // stage twiddle tables of the world's plans
     const float *S;          // [K][Wh][H] kernel spectra / (W H)
     const float *D;          // [nShells][Wh][H] shell-gradient spectra

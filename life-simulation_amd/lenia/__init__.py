@@ -135,6 +135,13 @@ class World:
                                                 g.ctypes.data, gA0.ctypes.data, C.byref(pg), 0))
         return grads_dict(pg, len(self.desc["kernels"]), gA0)
 
+    def autotune(self, n_steps: int = 20) -> float:
+        """time every launch shape the FFT plans allow, keep the fastest;
+        returns the tuned kernel ms per step (state and clock restored)"""
+        ms = C.c_double()
+        check(_abi.lib().lenia_world_autotune(self._h, n_steps, C.byref(ms)))
+        return ms.value
+
     def resolved(self) -> dict:
         d = _abi.WorldDesc()
         check(_abi.lib().lenia_world_get_desc(self._h, C.byref(d)))
@@ -174,6 +181,13 @@ class Batch:
 
     def step(self, n: int, graph_steps: int = 0):
         check(_abi.lib().lenia_batch_step(self._h, n, graph_steps))
+
+    def autotune(self, n_steps: int = 10) -> float:
+        """time every launch shape valid for all worlds, keep the fastest;
+        returns the tuned kernel ms per step (states and clocks restored)"""
+        ms = C.c_double()
+        check(_abi.lib().lenia_batch_autotune(self._h, n_steps, C.byref(ms)))
+        return ms.value
 
     def close(self):
         if self._h:

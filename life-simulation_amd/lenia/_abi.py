@@ -36,6 +36,7 @@ EXPORTS = [
     "lenia_world_get_desc", "lenia_world_pass_times",
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
     "lenia_rollout_forward", "lenia_rollout_backward",
+    "lenia_world_autotune", "lenia_batch_autotune",
 ]
 
 
@@ -146,6 +147,8 @@ def lib():
         "lenia_batch_destroy": (i32, [vp]),
         "lenia_batch_step": (i32, [vp, u32, u32]),
         "lenia_rollout_forward": (i32, [vp, u32, vp, vp, i32]),
+        "lenia_world_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
+        "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
     }
     for name, (res, args) in sig.items():

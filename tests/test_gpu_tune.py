@@ -1,0 +1,69 @@
+"""Launch-shape autotuning (lenia_world_autotune / lenia_batch_autotune):
+tuning changes only which threads run a butterfly, never the arithmetic, so a
+tuned world or batch gives the same bytes as an untuned one, and the tuning
+runs leave the state and step clock where they were."""
+import numpy as np
+import pytest
+
+from lenia_cases import config1, config4, sweep_worlds
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import lenia
+
+    c = lenia.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("desc", [config4(), config1(W=512, H=256)], ids=["256x256_K2", "512x256_K3"])
+def test_world_autotune_keeps_bits_and_state(ctx, desc):
+    import lenia
+
+    with lenia.World(ctx, desc) as plain, lenia.World(ctx, desc) as tuned:
+        for w in (plain, tuned):
+            w.reset(2024)
+            w.frame(0, 1)
+            w.step(5)
+        before = tuned.read()
+        ms = tuned.autotune(8)
+        after = tuned.read()
+        assert np.array_equal(before, after)
+        print(f"{desc['width']}x{desc['height']}: tuned {tuned.resolved()} {ms * 1e3:.1f} us/step, "
+              f"default {plain.resolved()}")
+        assert ms > 0
+        for w in (plain, tuned):
+            w.step(40)
+        assert plain.read().tobytes() == tuned.read().tobytes()
+        # the frame clock continued where it was: frame semantics still match
+        plain.frame(100, 2)
+        tuned.frame(100, 2)
+        assert plain.read().tobytes() == tuned.read().tobytes()
+
+
+def test_batch_autotune_keeps_bits(ctx):
+    import lenia
+
+    descs = sweep_worlds(16, seed=11)
+    seeds = [500 + i for i in range(len(descs))]
+    worlds = [lenia.World(ctx, d) for d in descs]
+    alone = [lenia.World(ctx, d) for d in descs]
+    try:
+        for ws in (worlds, alone):
+            for w, s in zip(ws, seeds):
+                w.reset(s)
+                w.frame(0, 1)
+        with lenia.Batch(ctx, worlds) as b:
+            ms = b.autotune(6)
+            print(f"batch of {len(descs)} tuned: {ms * 1e3:.1f} us/step")
+            b.step(16, 8)
+        for w in alone:
+            w.step(16)
+        for i, (w, a) in enumerate(zip(worlds, alone)):
+            assert w.read().tobytes() == a.read().tobytes(), i
+    finally:
+        for w in worlds + alone:
+            w.close()
