@@ -115,8 +115,9 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
 
 def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
     """differentiable-rollout world (256x256, two ring kernels R=16): wall time
-    of the T-step forward rollout and of its reverse pass (checkpoints every
-    `ckpt` steps, recompute included), host arrays in and out, best of reps"""
+    of the T-step forward rollout (which keeps checkpoints every `ckpt` steps)
+    and of its reverse pass (segment recompute included), host arrays in and
+    out, best of reps; also the reverse pass that reruns the forward sweep"""
     import numpy as np
 
     import lenia
@@ -132,21 +133,25 @@ def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
     with lenia.World(ctx, dict(desc, initMode=INIT_NONE)) as w:
         w.write(A0)
         w.autotune(10)
-        w.rollout(A0, T)
-        w.rollout_grad(A0, T, gT, ckpt)  # workspace, graphs, spectra
-        fwd, bwd = [], []
+        w.rollout(A0, T, ckpt)
+        w.rollout_grad(None, T, gT, ckpt)  # workspace, graphs, spectra
+        fwd, bwd, full = [], [], []
         for _ in range(reps):
             t0 = time.perf_counter()
-            w.rollout(A0, T)
+            w.rollout(A0, T, ckpt)  # keeps the checkpoints
             t1 = time.perf_counter()
-            w.rollout_grad(A0, T, gT, ckpt)
+            w.rollout_grad(None, T, gT, ckpt)  # recompute + reverse steps
             t2 = time.perf_counter()
+            w.rollout_grad(A0, T, gT, ckpt)  # forward sweep + recompute + reverse
+            t3 = time.perf_counter()
             fwd.append(t1 - t0)
             bwd.append(t2 - t1)
-    f, b = min(fwd) * 1e3, min(bwd) * 1e3
+            full.append(t3 - t2)
+    f, b, fb = min(fwd) * 1e3, min(bwd) * 1e3, min(full) * 1e3
     return {"world": "256x256 K=2 R=16 FFT", "T": T, "ckpt_every": ckpt,
             "forward_ms": round(f, 4), "backward_ms": round(b, 4),
             "forward_plus_adjoint_over_forward": round((f + b) / f, 3),
+            "backward_with_forward_sweep_ms": round(fb, 4),
             "mcell_steps_per_s_forward": round(256 * 256 * T / f / 1e3, 1)}
 
 

@@ -171,11 +171,16 @@ typedef struct lenia_param_grads {
     double dDt;
 } lenia_param_grads;
 
-/* T steps from A0; the world's state ends at AT */
-int lenia_rollout_forward(lenia_world *w, uint32_t T, const float *A0, float *AT, int onDevice);
-/* reverse pass of the same rollout for L with dL/dA_T = gAT: writes dL/dA0
- * (may be NULL) and the parameter gradients.  States are recomputed from
- * checkpoints every ckptEvery steps (0: floor(sqrt(T))). */
+/* T steps from A0; the world's state ends at AT.  The states at t = 0, c, 2c,
+ * ... (c = ckptEvery, 0: floor(sqrt(T))) are kept for a following
+ * lenia_rollout_backward. */
+int lenia_rollout_forward(lenia_world *w, uint32_t T, uint32_t ckptEvery, const float *A0, float *AT,
+                          int onDevice);
+/* reverse pass of the rollout from A0 for L with dL/dA_T = gAT: writes dL/dA0
+ * (may be NULL) and the parameter gradients.  The states of each segment of c
+ * steps are recomputed from its checkpoint.  A0 NULL reuses the checkpoints of
+ * the last lenia_rollout_forward of this world (same T and c, parameters
+ * unchanged since; else LENIA_ERR_STATE); otherwise the forward sweep runs first. */
 int lenia_rollout_backward(lenia_world *w, uint32_t T, uint32_t ckptEvery, const float *A0,
                            const float *gAT, float *gA0, lenia_param_grads *grads, int onDevice);
 

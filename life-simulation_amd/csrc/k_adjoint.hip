@@ -15,16 +15,16 @@
 // In the spectral domain the adjoint convolution is the conjugate-spectrum
 // product: multiply by the (real, even) kernel spectrum and transform back.
 //
-// One reverse step is four launches (row transforms are along x on row pairs
-// packed as one complex row, column transforms along y on the half spectrum):
-//   k_adj_cols<POT>  F_t (row spectrum of A_t) -> Ahat (2D spectrum),
-//                    G_k = column inverse of S_k Ahat
-//   k_adj_mid        row inverse of G_k -> u_k; per cell gp, gu_k and the
-//                    parameter partials; row forward of gu_k -> FG_k
-//   k_adj_cols<ADJ>  column forward of FG_k; shell partials against Ahat;
+// The recompute of a segment runs the normal step kernels with their save
+// pointers set (FftWorldDev::saveA/saveU/saveAhat), so each step leaves its
+// input state A_t, its potentials u_k and the 2D spectrum Ahat_t.  One reverse
+// step is then three launches (row transforms along x on row pairs packed as
+// one complex row, column transforms along y on the half spectrum):
+//   k_adj_mid        per cell gp, gu_k and the parameter partials from the
+//                    saved u_k and A_t; row forward of gu_k -> FG_k
+//   k_adj_cols       column forward of FG_k; shell partials against Ahat_t;
 //                    G1 = column inverse of sum_k S_k FG_k^
-//   k_adj_edge       g_t = gp + row inverse of G1, and the row spectrum
-//                    F_{t-1} of the previous state for the next reverse step
+//   k_adj_edge       g_t = gp + row inverse of G1
 // The kernels reuse the LDS FFT of the forward path (same plans, same
 // butterflies).  Each workgroup adds its partial sums to its own FP64 slots
 // (one row pair or one spectrum column per workgroup, every step in order),
@@ -79,12 +79,10 @@ __device__ __forceinline__ void split_row_pair(const float2 *Z, int base, float 
 }
 
 // ------------------------------------------------------------------ rows: edge
-// slot 0: g = gpb + row inverse of G1 (if G1); slot 1: row spectrum of the
-// state Anext (if Anext).  The inverse runs as a forward transform of the
-// conjugate, so both slots share one pass.  One row pair per workgroup.
+// g = gpb + row inverse of G1, one row pair per workgroup
 template <int T>
 __global__ __launch_bounds__(T) void k_adj_edge(AdjArgs a, const float *G1, const float *gpb,
-                                                float *gOut, const float *Anext, float *Fnext) {
+                                                float *gOut) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
     const int W = a.W, H2 = a.H / 2;
@@ -92,37 +90,25 @@ __global__ __launch_bounds__(T) void k_adj_edge(AdjArgs a, const float *G1, cons
     float2 *tw = smem;
     float2 *Z = smem + W;
     load_twiddles(tw, a.twW, W, tid, T);
-    // slot 0 holds the conjugate: inverse = conj(forward(conj))
-    if (G1) {
-        gather_row_pair<true>(Z, 0, G1, W, H2, rp, tid, T);
-    } else {
-        for (int x = tid; x < W; x += T) Z[sw(x)] = make_float2(0.f, 0.f);
-    }
-    const LENIA_GLOBAL float *an = gp(Anext);
-    for (int x = tid; x < W; x += T)
-        Z[sw(W + x)] = Anext ? make_float2(an[(size_t)(2 * rp) * W + x], an[(size_t)(2 * rp + 1) * W + x])
-                             : make_float2(0.f, 0.f);
+    gather_row_pair<false>(Z, 0, G1, W, H2, rp, tid, T);
     __syncthreads();
-    fft_lds<false, kAV, T>(Z, a.planW, 2, tw, tid);
-    if (G1) {
-        const LENIA_GLOBAL float *ad = gp(gpb);
-        LENIA_GLOBAL float *go = gp(gOut);
-        for (int x = tid; x < W; x += T) {
-            float2 z = Z[sw(x)];  // conj of the inverse: real parts unchanged, imag negated
-            const size_t i0 = (size_t)(2 * rp) * W + x;
-            go[i0] = ad[i0] + z.x;
-            go[i0 + W] = ad[i0 + W] - z.y;
-        }
+    fft_lds<true, kAV, T>(Z, a.planW, 1, tw, tid);
+    const LENIA_GLOBAL float *ad = gp(gpb);
+    LENIA_GLOBAL float *go = gp(gOut);
+    for (int x = tid; x < W; x += T) {
+        float2 z = Z[sw(x)];
+        const size_t i0 = (size_t)(2 * rp) * W + x;
+        go[i0] = ad[i0] + z.x;
+        go[i0 + W] = ad[i0 + W] + z.y;
     }
-    if (Anext) split_row_pair(Z, W, Fnext, W, H2, rp, tid, T);
 }
 
 // ------------------------------------------------------------------ rows: middle
-// row inverse of the K slots of Gk -> u_k rows; cells -> gp (gpb), gu_k;
-// row forward of gu_k -> FG.  Partials of dt, mu_k, sigma_k, w_k are added to
+// saved potentials U [K][H][W] and state A -> per cell gp (gpb), gu_k; row
+// forward of gu_k -> FG.  Partials of dt, mu_k, sigma_k, w_k are added to
 // accRows[rp][v] (FP64).
 template <int T>
-__global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *Gk, const float *A,
+__global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const float *A,
                                                const float *g, float *gpb, float *FG,
                                                double *accRows) {
     extern __shared__ float2 smem[];
@@ -134,10 +120,6 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *Gk, const
     float2 *Z = smem + W;
     float *red = reinterpret_cast<float *>(Z + ((K * W + 31) & ~31));
     load_twiddles(tw, a.twW, W, tid, T);
-    for (int k = 0; k < K; ++k)
-        gather_row_pair<false>(Z, k * W, Gk + k * sStride * 2, W, H2, rp, tid, T);
-    __syncthreads();
-    fft_lds<true, kAV, T>(Z, a.planW, K, tw, tid);
     const GrowthParams &gr = *a.gDev;
     const float dt = gr.dt;
     float mu[4], bs[4], wk[4], sg[4];
@@ -153,12 +135,15 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *Gk, const
     for (int v = 0; v < 13; ++v) vals[v] = 0.0f;
     const LENIA_GLOBAL float *Ag = gp(A);
     const LENIA_GLOBAL float *gg = gp(g);
+    const LENIA_GLOBAL float *Ug = gp(U);
     LENIA_GLOBAL float *gpo = gp(gpb);
+    const size_t n = (size_t)W * H;
     for (int x = tid; x < W; x += T) {
         float2 u2[4], gu2[4];
+        const size_t i0 = (size_t)(2 * rp) * W + x;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            u2[k] = k < K ? Z[sw(k * W + x)] : make_float2(0.f, 0.f);
+            u2[k] = k < K ? make_float2(Ug[k * n + i0], Ug[k * n + i0 + W]) : make_float2(0.f, 0.f);
             gu2[k] = make_float2(0.f, 0.f);
         }
 #pragma unroll
@@ -211,12 +196,11 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *Gk, const
 }
 
 // ------------------------------------------------------------------ columns
-// POT: F -> (store the forward column spectrum Ahat) -> x S_k -> inverse -> G[k]
-// ADJ: FG[k] (k < K) -> forward -> spectral partial sums for the shell
-//      gradients against Ahat (added to accCols[c][i]) -> Y = sum_k S_k FG^[k]
-//      -> inverse -> G[0]
-template <int T, bool ADJ>
-__global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, float *Ahat,
+// FG[k] (k < K) -> column forward -> spectral partial sums for the shell
+// gradients against Ahat (added to accCols[c][i]) -> Y = sum_k S_k FG^[k] ->
+// column inverse -> G
+template <int T>
+__global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, const float *Ahat,
                                                 float *G, double *accCols) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
@@ -224,31 +208,15 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, flo
     const int c = blockIdx.x;  // spectrum column kx
     const size_t sStride = (size_t)Wh * H;
     float2 *tw = smem;
-    float2 *X = smem + H;  // ADJ: K slots; POT: 1 slot
+    float2 *X = smem + H;  // K slots
     load_twiddles(tw, a.twH, H, tid, T);
-    const int nin = ADJ ? K : 1;
-    for (int e = tid; e < nin * H; e += T) {
+    for (int e = tid; e < K * H; e += T) {
         int s = e / H, ky = e - s * H;
         X[sw(e)] = gld2(gp(reinterpret_cast<const float2 *>(Fin)) + s * sStride + (size_t)c * H + ky);
     }
     __syncthreads();
-    fft_lds<false, kAV, T>(X, a.planH, nin, tw, tid);
-    float2 *Y = X + ((nin * H + 31) & ~31);
-    if (!ADJ) {
-        for (int e = tid; e < H; e += T)
-            gst2(gp(reinterpret_cast<float2 *>(Ahat)) + (size_t)c * H + e, X[sw(e)]);
-        for (int e = tid; e < K * H; e += T) {
-            int k = e / H, ky = e - k * H;
-            Y[sw(e)] = cscale(X[sw(ky)], gp(a.S)[k * sStride + (size_t)c * H + ky]);
-        }
-        __syncthreads();
-        fft_lds<true, kAV, T>(Y, a.planH, K, tw, tid);
-        for (int e = tid; e < K * H; e += T) {
-            int k = e / H, ky = e - k * H;
-            gst2(gp(reinterpret_cast<float2 *>(G)) + k * sStride + (size_t)c * H + ky, Y[sw(e)]);
-        }
-        return;
-    }
+    fft_lds<false, kAV, T>(X, a.planH, K, tw, tid);
+    float2 *Y = X + ((K * H + 31) & ~31);
     // shell partial sums: thread-strided over ky, then a fixed-order LDS tree
     const float wt = (c == 0 || c == Wh - 1) ? 1.0f : 2.0f;
     const int nb = a.nShells;  // total (k, j) pairs, <= 16
@@ -336,53 +304,62 @@ static hipError_t by_threads(int T, F f) {
 static inline int r32(int n) { return (n + 31) & ~31; }
 
 size_t adj_lds_rows(const AdjArgs &a, int T) {
-    const size_t edge = (size_t)(a.W + r32(2 * a.W)) * sizeof(float2);
-    const size_t mid = (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
-    return edge > mid ? edge : mid;
+    // mid: twiddles | Z [K][W] | reduction [T];  edge: twiddles | Z [W]
+    return (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
 }
 
 size_t adj_lds_cols(const AdjArgs &a, int T) {
-    // POT: twiddles | X [1][H] | Y [K][H];  ADJ: twiddles | X [K][H] | Y [H] | reduction [T]
-    const size_t pot = (size_t)(a.H + r32(a.H) + r32(a.K * a.H)) * sizeof(float2);
-    const size_t adj = (size_t)(a.H + r32(a.K * a.H) + r32(a.H)) * sizeof(float2) + (size_t)T * sizeof(float);
-    return pot > adj ? pot : adj;
+    // twiddles | X [K][H] | Y [H] | reduction [T]
+    return (size_t)(a.H + r32(a.K * a.H) + r32(a.H)) * sizeof(float2) + (size_t)T * sizeof(float);
 }
 
 hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
-                           const float *Anext, float *Fnext, hipStream_t s) {
-    const size_t lds = (size_t)(a.W + r32(2 * a.W)) * sizeof(float2);
+                           hipStream_t s) {
+    const size_t lds = (size_t)(a.W + r32(a.W)) * sizeof(float2);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        hipLaunchKernelGGL(k_adj_edge<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, G1, gpb, gOut, Anext,
-                           Fnext);
+        hipLaunchKernelGGL(k_adj_edge<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, G1, gpb, gOut);
     });
 }
 
-hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *Gk, const float *A, const float *g,
+hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *U, const float *A, const float *g,
                           float *gpb, float *FG, double *accRows, hipStream_t s) {
-    const size_t lds = (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
+    const size_t lds = adj_lds_rows(a, T);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        hipLaunchKernelGGL(k_adj_mid<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, Gk, A, g, gpb, FG,
+        hipLaunchKernelGGL(k_adj_mid<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, U, A, g, gpb, FG,
                            accRows);
     });
 }
 
-hipError_t launch_adj_cols(const AdjArgs &a, int T, bool adj, const float *Fin, float *Ahat,
-                           float *G, double *accCols, hipStream_t s) {
-    const int nin = adj ? a.K : 1;
-    size_t lds = (size_t)(a.H + r32(nin * a.H) + (adj ? r32(a.H) : r32(a.K * a.H))) * sizeof(float2) +
-                 (adj ? (size_t)T * sizeof(float) : 0);
+hipError_t launch_adj_cols(const AdjArgs &a, int T, const float *Fin, const float *Ahat, float *G,
+                           double *accCols, hipStream_t s) {
+    const size_t lds = adj_lds_cols(a, T);
     const int Wh = a.W / 2 + 1;
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        if (adj)
-            hipLaunchKernelGGL((k_adj_cols<TT, true>), dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat, G,
-                               accCols);
-        else
-            hipLaunchKernelGGL((k_adj_cols<TT, false>), dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat, G,
-                               accCols);
+        hipLaunchKernelGGL(k_adj_cols<TT>, dim3(Wh), dim3(TT), lds, s, a, Fin, Ahat, G, accCols);
     });
+}
+
+// copies of the world record with save pointers for c recompute steps:
+// step j saves into saveA + j n, saveU + j K n, saveAhat + j spec
+__global__ void k_adj_records(const FftWorldDev *src, FftWorldDev *dst, int c, float *saveA,
+                              float *saveU, float *saveAhat, size_t n, size_t spec) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= c) return;
+    FftWorldDev r = *src;
+    r.saveA = saveA + (size_t)j * n;
+    r.saveU = saveU + (size_t)j * r.K * n;
+    r.saveAhat = saveAhat + (size_t)j * spec;
+    dst[j] = r;
+}
+
+hipError_t launch_adj_records(const FftWorldDev *src, FftWorldDev *dst, int c, float *saveA,
+                              float *saveU, float *saveAhat, size_t n, size_t spec, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_records, dim3((c + 63) / 64), dim3(64), 0, s, src, dst, c, saveA, saveU,
+                       saveAhat, n, spec);
+    return hipGetLastError();
 }
 
 hipError_t launch_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,

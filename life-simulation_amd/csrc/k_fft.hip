@@ -228,6 +228,16 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                         float2 z = Z[sw(k * nq + q)];
                         sa += growth_term(g, k, z.x, ja);
                         sb += growth_term(g, k, z.y, jb);
+                        if (wd.saveU) {
+                            LENIA_GLOBAL float *su = gp(wd.saveU) + (size_t)k * W * H + (size_t)ya * W + x;
+                            su[0] = z.x;
+                            su[W] = z.y;
+                        }
+                    }
+                    if (wd.saveA) {
+                        LENIA_GLOBAL float *sa0 = gp(wd.saveA) + (size_t)ya * W + x;
+                        sa0[0] = ar0[i];
+                        sa0[W] = ar1[i];
                     }
                     if (EXTRAS) {
                         ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
@@ -278,6 +288,11 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                         }
                         sa[i] += growth_term(g, k, z.x, ja);
                         sb[i] += growth_term(g, k, z.y, jb);
+                        if (wd.saveU) {
+                            LENIA_GLOBAL float *su = gp(wd.saveU) + (size_t)k * W * H + (size_t)ya * W + x;
+                            su[0] = z.x;
+                            su[W] = z.y;
+                        }
                     }
                 }
                 __syncthreads();
@@ -289,6 +304,11 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
                     int ya = 2 * (rp0 + p);
                     float a2, b2;
+                    if (wd.saveA) {
+                        LENIA_GLOBAL float *sa0 = gp(wd.saveA) + (size_t)ya * W + x;
+                        sa0[0] = ar0[i];
+                        sa0[W] = ar1[i];
+                    }
                     if (EXTRAS) {
                         a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
                         b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
@@ -362,6 +382,10 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             }
             __syncthreads();
             fft_lds<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            if (wd.saveAhat) {
+                LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+                for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
+            }
             // all reads of the forward spectrum first, then the scaled copies
             float2 xv[kMaxV];
 #pragma unroll
@@ -399,6 +423,14 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
         for (int i = 0; i < kMaxV; ++i) {
             int e = tid + i * T;
             xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
+        }
+        if (wd.saveAhat) {
+            LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) gst2(sv + e, xs[i]);
+            }
         }
         for (int k = 0; k < K; ++k) {
 #pragma unroll

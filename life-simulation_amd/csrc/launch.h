@@ -60,13 +60,15 @@ size_t mfma_lds_bytes(int h, bool split);
 
 // adjoint (k_adjoint.hip)
 hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
-                           const float *Anext, float *Fnext, hipStream_t s);
-hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *Gk, const float *A, const float *g,
+                           hipStream_t s);
+hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *U, const float *A, const float *g,
                           float *gpb, float *FG, double *accRows, hipStream_t s);
-hipError_t launch_adj_cols(const AdjArgs &a, int T, bool adj, const float *Fin, float *Ahat,
-                           float *G, double *accCols, hipStream_t s);
+hipError_t launch_adj_cols(const AdjArgs &a, int T, const float *Fin, const float *Ahat, float *G,
+                           double *accCols, hipStream_t s);
 hipError_t launch_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,
                             int nCols, int nvCols, double *acc, hipStream_t s);
+hipError_t launch_adj_records(const FftWorldDev *src, FftWorldDev *dst, int c, float *saveA,
+                              float *saveU, float *saveAhat, size_t n, size_t spec, hipStream_t s);
 size_t adj_lds_rows(const AdjArgs &a, int T);
 size_t adj_lds_cols(const AdjArgs &a, int T);
 

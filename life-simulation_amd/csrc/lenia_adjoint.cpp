@@ -4,14 +4,17 @@
 // are exactly the library's own, so the reverse pass differentiates the
 // arithmetic that produced the states.
 //
-// Reverse pass: a forward sweep stores the states at t = 0, c, 2c, ...; then,
-// segment by segment from the last, the states of the segment are recomputed
-// bit-exactly from its checkpoint with the normal step kernels and the steps
-// are reversed from the segment's end to its start (k_adjoint.hip, four
-// launches per step).  The whole sequence -- sweep, recomputes, reverse steps
-// and the final sums -- is captured once per (T, c) into a hipGraph and
-// replayed; every kernel reads the growth parameters from the world's device
-// record, so parameter updates between calls keep the graph valid.
+// Forward: the states at t = 0, c, 2c, ... are kept as checkpoints while the
+// rollout runs.  Reverse: segment by segment from the last, the segment's
+// steps are recomputed bit-exactly from its checkpoint with the normal step
+// kernels, which (through the save pointers of a copy of the world record)
+// also leave each step's input state, potentials and 2D spectrum; the steps
+// are then reversed from the segment's end to its start with three launches
+// each (k_adjoint.hip).  Both sequences are captured once per (T, c) into
+// hipGraphs and replayed; every kernel reads the growth parameters from the
+// world's device record, so parameter updates between calls keep the graphs
+// valid.  A reverse call can reuse the checkpoints of the last forward call
+// (same T, c and parameters) instead of running the forward sweep again.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,23 +28,36 @@
 using namespace lenia;
 
 struct AdjWorkspace {
-    size_t n = 0;  // cells
+    size_t n = 0;     // cells
+    size_t spec = 0;  // floats of one complex half spectrum
     int K = 0, T = 0, c = 0, nShells = 0;
-    float *ckpt = nullptr, *seg = nullptr;
-    float *Ftmp = nullptr, *Ahat = nullptr, *Gk = nullptr, *FG = nullptr, *G1 = nullptr,
-          *gpb = nullptr, *g = nullptr, *D = nullptr;
+    float *ckpt = nullptr;  // [ceil(T/c)][n] states at t = 0, c, 2c, ...
+    // two sets of c slots: segment sg recomputes into set sg % 2 while the
+    // reverse steps of segment sg + 1 read the other set
+    float *saveA = nullptr, *saveU = nullptr, *saveAhat = nullptr;  // [2c] slots
+    FftWorldDev *recs = nullptr;  // [2c] world records with save pointers
+    hipStream_t side = nullptr;   // the recompute branch of the captured graph
+    hipEvent_t fork = nullptr, join = nullptr;
+    float *FG = nullptr, *G1 = nullptr, *gpb = nullptr, *g = nullptr, *D = nullptr;
     double *accRows = nullptr, *accCols = nullptr, *acc = nullptr;
     int rowT = 256, colT = 256;
     bool dValid = false;  // D holds the shell spectra of the current kernels
     AdjArgs args{};
-    hipGraphExec_t graph = nullptr;
-    uint32_t graphShapes = 0;  // the world's shapeVersion the graph was recorded with
+    hipGraphExec_t fwdGraph = nullptr, bwdGraph = nullptr;
+    uint32_t graphShapes = 0;  // the world's shapeVersion the graphs were recorded with
+    bool ckptValid = false;    // ckpt holds a forward sweep ...
+    uint32_t ckptVersion = 0;  // ... run with this world descriptor version
 };
 
 static void ws_free(lenia_ctx *ctx, AdjWorkspace *ws) {
-    if (ws->graph) (void)hipGraphExecDestroy(ws->graph);
-    float *ptrs[] = {ws->ckpt, ws->seg, ws->Ftmp, ws->Ahat, ws->Gk, ws->FG, ws->G1, ws->gpb, ws->g, ws->D};
+    if (ws->fwdGraph) (void)hipGraphExecDestroy(ws->fwdGraph);
+    if (ws->bwdGraph) (void)hipGraphExecDestroy(ws->bwdGraph);
+    float *ptrs[] = {ws->ckpt, ws->saveA, ws->saveU, ws->saveAhat, ws->FG, ws->G1, ws->gpb, ws->g, ws->D};
     for (float *p : ptrs) dev_free(ctx, p);
+    dev_free(ctx, ws->recs);
+    if (ws->fork) (void)hipEventDestroy(ws->fork);
+    if (ws->join) (void)hipEventDestroy(ws->join);
+    if (ws->side) (void)hipStreamDestroy(ws->side);
     dev_free(ctx, ws->accRows);
     dev_free(ctx, ws->accCols);
     dev_free(ctx, ws->acc);
@@ -153,6 +169,7 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
         ws = new AdjWorkspace();
         w->adj = ws;
         ws->n = n;
+        ws->spec = spec;
         ws->K = K;
         ws->T = (int)T;
         ws->c = (int)c;
@@ -177,10 +194,16 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
         a.twH = w->twH;
         a.S = w->S;
         int rc = dev_alloc(ctx, (void **)&ws->ckpt, sizeof(float) * n * nck);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->seg, sizeof(float) * n * c);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->Ftmp, sizeof(float) * spec);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->Ahat, sizeof(float) * spec);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->Gk, sizeof(float) * spec * K);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveA, sizeof(float) * n * 2 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveU, sizeof(float) * n * K * 2 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveAhat, sizeof(float) * spec * 2 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->recs, sizeof(FftWorldDev) * 2 * c);
+        if (!rc) {
+            hipError_t e = hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&ws->join, hipEventDisableTiming);
+            if (e != hipSuccess) rc = hip_fail(e, "rollout streams");
+        }
         if (!rc) rc = dev_alloc(ctx, (void **)&ws->FG, sizeof(float) * spec * K);
         if (!rc) rc = dev_alloc(ctx, (void **)&ws->G1, sizeof(float) * spec);
         if (!rc) rc = dev_alloc(ctx, (void **)&ws->gpb, sizeof(float) * n);
@@ -194,15 +217,14 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
             return rc;
         }
         a.D = ws->D;
-        // workgroup sizes: about 4 values per thread for the largest batch of
-        // transforms (max(K, 2) rows, K columns) -- short stages over many
-        // threads, since one reverse step has only H/2 + W/2+1 workgroups --
-        // within <= 16 values per thread and 64 KB of LDS
+        // workgroup sizes: about 2 values per thread for the K transforms of
+        // a row pair or column -- short stages over many threads, since one
+        // reverse step has only H/2 + W/2+1 workgroups -- within <= 16 values
+        // per thread and 64 KB of LDS
         const int mv = fft_max_values();
-        const int nr = std::max(K, 2);
         auto pick = [&](const FftPlan &p, int nt, bool rows) {
             int want = 64;
-            while (want < 512 && want * 4 < nt * p.n) want *= 2;
+            while (want < 512 && want * 2 < nt * p.n) want *= 2;
             for (int T2 = want; T2 <= 512; T2 *= 2)
                 if (plan_fits(p, nt, T2, mv) && plan_fits(p, 1, T2, mv) &&
                     (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= 65536)
@@ -213,7 +235,7 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
                     return T2;
             return 0;
         };
-        ws->rowT = pick(w->planW, nr, true);
+        ws->rowT = pick(w->planW, K, true);
         ws->colT = pick(w->planH, K, false);
         if (!ws->rowT || !ws->colT) {
             world_free_adjoint(w);
@@ -234,133 +256,192 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
 static hipMemcpyKind kind_in(int onDevice) { return onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice; }
 static hipMemcpyKind kind_out(int onDevice) { return onDevice ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost; }
 
-extern "C" int lenia_rollout_forward(lenia_world *w, uint32_t T, const float *A0, float *AT,
-                                     int onDevice) {
-    if (!w || !A0 || !AT) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_forward: NULL argument");
+static uint32_t ckpt_spacing(uint32_t T, uint32_t ckptEvery) {
+    return ckptEvery ? std::min(ckptEvery, T) : (uint32_t)std::max(1.0, std::floor(std::sqrt((double)T)));
+}
+
+// state := src (device) and its row spectrum, on stream s
+static hipError_t load_state(lenia_world *w, const float *src, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(w->current_state(), src, sizeof(float) * w->adj->n,
+                                  hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess)
+        e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles,
+                            w->nRowTiles, w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, s);
+    return e;
+}
+
+// forward sweep from ckpt[0]: checkpoints at t = c, 2c, ...; the world ends at A_T
+static hipError_t enqueue_forward(lenia_world *w) {
+    AdjWorkspace *ws = w->adj;
+    hipStream_t s = w->ctx->stream;
+    const size_t n = ws->n;
+    const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
+    hipError_t e = load_state(w, ws->ckpt, s);
+    for (uint32_t k = 1; k < nck && e == hipSuccess; ++k) {
+        for (uint32_t i = 0; i < c && e == hipSuccess; ++i) e = enqueue_step(w, nullptr);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(ws->ckpt + n * k, w->current_state(), sizeof(float) * n,
+                               hipMemcpyDeviceToDevice, s);
+    }
+    for (uint32_t t = (nck - 1) * c; t < T && e == hipSuccess; ++t) e = enqueue_step(w, nullptr);
+    return e;
+}
+
+// recompute segment sg (steps t0 .. t0+m-1) from its checkpoint on stream s;
+// step t0 + j leaves A, u_k and Ahat in slot j of set sg % 2
+static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
+    AdjWorkspace *ws = w->adj;
+    const uint32_t T = ws->T, c = ws->c, t0 = sg * c, m = std::min(T, t0 + c) - t0;
+    const FftWorldDev *recs = ws->recs + (size_t)(sg % 2) * c;
+    hipError_t e = load_state(w, ws->ckpt + ws->n * sg, s);
+    for (uint32_t j = 0; j < m && e == hipSuccess; ++j) {
+        e = launch_fft_cols(w->colT, w->colKpar, recs + j, w->colTiles, w->nColTiles, w->nColTiles,
+                            w->ldsCols, s);
+        if (e == hipSuccess)
+            e = launch_fft_rows(1, w->rowT, false, w->rowKpar, recs + j, w->rowTiles, w->nRowTiles,
+                                w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, s);
+    }
+    return e;
+}
+
+// reverse the steps of segment sg from its saves, last step first
+static hipError_t reverse_segment(lenia_world *w, int sg, hipStream_t s) {
+    AdjWorkspace *ws = w->adj;
+    const AdjArgs &a = ws->args;
+    const size_t n = ws->n;
+    const uint32_t T = ws->T, c = ws->c, t0 = sg * c, m = std::min(T, t0 + c) - t0;
+    const size_t base = (size_t)(sg % 2) * c;
+    hipError_t e = hipSuccess;
+    for (int j = (int)m - 1; j >= 0 && e == hipSuccess; --j) {
+        const size_t slot = base + j;
+        e = launch_adj_mid(a, ws->rowT, ws->saveU + slot * a.K * n, ws->saveA + slot * n, ws->g,
+                           ws->gpb, ws->FG, ws->accRows, s);
+        if (e == hipSuccess)
+            e = launch_adj_cols(a, ws->colT, ws->FG, ws->saveAhat + slot * ws->spec, ws->G1,
+                                ws->accCols, s);
+        if (e == hipSuccess) e = launch_adj_edge(a, ws->rowT, ws->G1, ws->gpb, ws->g, s);
+    }
+    return e;
+}
+
+// the reverse pass from the checkpoints and g = dL/dA_T: leaves the
+// parameter sums in acc and dL/dA0 in g, the world at A0.  The recompute of
+// segment sg - 1 (side stream) runs beside the reverse steps of segment sg.
+static hipError_t enqueue_backward(lenia_world *w) {
+    AdjWorkspace *ws = w->adj;
+    const AdjArgs &a = ws->args;
+    hipStream_t s = w->ctx->stream;
+    const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
+    const int H2 = a.H / 2, Wh = a.W / 2 + 1;
+    // records with save pointers (refreshed from the world record every call,
+    // so parameter updates are seen) and zeroed FP64 slots (fill kernels: graph-safe)
+    hipError_t e = launch_adj_records((const FftWorldDev *)w->devDesc, ws->recs, (int)(2 * c), ws->saveA,
+                                      ws->saveU, ws->saveAhat, ws->n, ws->spec, s);
+    if (e == hipSuccess) e = launch_fill(reinterpret_cast<float *>(ws->accRows), (size_t)32 * H2, 0.0f, s);
+    if (e == hipSuccess) e = launch_fill(reinterpret_cast<float *>(ws->accCols), (size_t)32 * Wh, 0.0f, s);
+    if (e == hipSuccess) e = recompute_segment(w, (int)nck - 1, s);
+    for (int sg = (int)nck - 1; sg >= 0 && e == hipSuccess; --sg) {
+        if (sg > 0) {
+            e = hipEventRecord(ws->fork, s);
+            if (e == hipSuccess) e = hipStreamWaitEvent(ws->side, ws->fork, 0);
+            if (e == hipSuccess) e = recompute_segment(w, sg - 1, ws->side);
+            if (e == hipSuccess) e = hipEventRecord(ws->join, ws->side);
+        }
+        if (e == hipSuccess) e = reverse_segment(w, sg, s);
+        if (e == hipSuccess && sg > 0) e = hipStreamWaitEvent(s, ws->join, 0);
+    }
+    if (e == hipSuccess)
+        e = launch_adj_final(ws->accRows, H2, 1 + 3 * a.K, ws->accCols, Wh, a.nShells, ws->acc, s);
+    if (e == hipSuccess) e = load_state(w, ws->ckpt, s);
+    return e;
+}
+
+static int capture(lenia_world *w, hipError_t (*body)(lenia_world *), hipGraphExec_t *out) {
+    hipStream_t s = w->ctx->stream;
+    if (*out) (void)hipGraphExecDestroy(*out);
+    *out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "rollout graph capture");
+    hipError_t eb = body(w);
+    e = hipStreamEndCapture(s, &graph);
+    if (eb != hipSuccess) e = eb;
+    if (e == hipSuccess) e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return hip_fail(e, "rollout graph");
+    }
+    return LENIA_OK;
+}
+
+// both graphs recorded with the world's current launch shapes
+static int ensure_graphs(lenia_world *w) {
+    AdjWorkspace *ws = w->adj;
+    if (ws->fwdGraph && ws->bwdGraph && ws->graphShapes == w->shapeVersion) return LENIA_OK;
+    int rc = capture(w, enqueue_forward, &ws->fwdGraph);
+    if (!rc) rc = capture(w, enqueue_backward, &ws->bwdGraph);
+    if (!rc) ws->graphShapes = w->shapeVersion;
+    return rc;
+}
+
+extern "C" int lenia_rollout_forward(lenia_world *w, uint32_t T, uint32_t ckptEvery, const float *A0,
+                                     float *AT, int onDevice) {
+    if (!w || !A0 || !AT || T == 0) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_forward: bad argument");
     int rc = check_differentiable(w);
     if (rc) return rc;
     lenia_ctx *ctx = w->ctx;
     (void)hipSetDevice(ctx->device);
-    const size_t n = (size_t)w->desc.width * w->desc.height;
-    hipError_t e = hipMemcpyAsync(w->current_state(), A0, sizeof(float) * n, kind_in(onDevice), ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "rollout input");
-    w->needsInit = false;
-    w->specValid = false;
-    rc = world_launch_steps(w, T);
+    rc = ws_prepare(w, T, ckpt_spacing(T, ckptEvery));
+    if (!rc) rc = ensure_graphs(w);
     if (rc) return rc;
-    e = hipMemcpyAsync(AT, w->current_state(), sizeof(float) * n, kind_out(onDevice), ctx->stream);
+    AdjWorkspace *ws = w->adj;
+    const size_t bytes = sizeof(float) * ws->n;
+    ws->ckptValid = false;
+    hipError_t e = hipMemcpyAsync(ws->ckpt, A0, bytes, kind_in(onDevice), ctx->stream);
+    if (e == hipSuccess) e = hipGraphLaunch(ws->fwdGraph, ctx->stream);
+    w->hostSteps += T;
+    w->needsInit = false;
+    w->specValid = true;
+    if (e == hipSuccess) e = hipMemcpyAsync(AT, w->current_state(), bytes, kind_out(onDevice), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? LENIA_OK : hip_fail(e, "rollout output");
-}
-
-// state := src (device) and its row spectrum, on the context stream
-static hipError_t load_state(lenia_world *w, const float *src) {
-    lenia_ctx *ctx = w->ctx;
-    hipError_t e = hipMemcpyAsync(w->current_state(), src, sizeof(float) * w->adj->n,
-                                  hipMemcpyDeviceToDevice, ctx->stream);
-    if (e == hipSuccess)
-        e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles,
-                            w->nRowTiles, w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
-    return e;
-}
-
-// the whole reverse pass for the workspace's (T, c), inputs already in
-// ckpt[0] (A0) and g (dL/dA_T); leaves the parameter sums in acc and dL/dA0 in g
-static hipError_t enqueue_backward(lenia_world *w) {
-    AdjWorkspace *ws = w->adj;
-    const AdjArgs &a = ws->args;
-    lenia_ctx *ctx = w->ctx;
-    hipStream_t s = ctx->stream;
-    const size_t n = ws->n, bytes = sizeof(float) * n;
-    const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
-    const int H2 = a.H / 2, Wh = a.W / 2 + 1;
-    // zero the per-workgroup FP64 slots (a fill kernel: graph-safe)
-    hipError_t e = launch_fill(reinterpret_cast<float *>(ws->accRows), (size_t)32 * H2, 0.0f, s);
-    if (e == hipSuccess) e = launch_fill(reinterpret_cast<float *>(ws->accCols), (size_t)32 * Wh, 0.0f, s);
-    // forward sweep: checkpoints at t = c, 2c, ...
-    if (e == hipSuccess) e = load_state(w, ws->ckpt);
-    for (uint32_t k = 1; k < nck && e == hipSuccess; ++k) {
-        for (uint32_t i = 0; i < c && e == hipSuccess; ++i) e = enqueue_step(w, nullptr);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(ws->ckpt + n * k, w->current_state(), bytes, hipMemcpyDeviceToDevice, s);
-    }
-    for (int sg = (int)nck - 1; sg >= 0 && e == hipSuccess; --sg) {
-        const uint32_t t0 = sg * c, t1 = std::min(T, t0 + c);
-        // recompute the segment's states A_t0 .. A_t1-1 into seg
-        e = load_state(w, ws->ckpt + n * sg);
-        if (e == hipSuccess) e = hipMemcpyAsync(ws->seg, ws->ckpt + n * sg, bytes, hipMemcpyDeviceToDevice, s);
-        for (uint32_t t = t0 + 1; t < t1 && e == hipSuccess; ++t) {
-            e = enqueue_step(w, nullptr);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(ws->seg + n * (t - t0), w->current_state(), bytes, hipMemcpyDeviceToDevice, s);
-        }
-        // row spectrum of the segment's last state, then the reverse steps
-        if (e == hipSuccess)
-            e = launch_adj_edge(a, ws->rowT, nullptr, nullptr, nullptr, ws->seg + n * (t1 - 1 - t0), ws->Ftmp, s);
-        for (int t = (int)t1 - 1; t >= (int)t0 && e == hipSuccess; --t) {
-            const float *At = ws->seg + n * (t - t0);
-            const float *Aprev = t > (int)t0 ? ws->seg + n * (t - 1 - t0) : nullptr;
-            e = launch_adj_cols(a, ws->colT, false, ws->Ftmp, ws->Ahat, ws->Gk, nullptr, s);
-            if (e == hipSuccess)
-                e = launch_adj_mid(a, ws->rowT, ws->Gk, At, ws->g, ws->gpb, ws->FG, ws->accRows, s);
-            if (e == hipSuccess)
-                e = launch_adj_cols(a, ws->colT, true, ws->FG, ws->Ahat, ws->G1, ws->accCols, s);
-            if (e == hipSuccess)
-                e = launch_adj_edge(a, ws->rowT, ws->G1, ws->gpb, ws->g, Aprev, ws->Ftmp, s);
-        }
-    }
-    if (e == hipSuccess)
-        e = launch_adj_final(ws->accRows, H2, 1 + 3 * a.K, ws->accCols, Wh, a.nShells, ws->acc, s);
-    // leave the world at A0 with a valid spectrum
-    if (e == hipSuccess) e = load_state(w, ws->ckpt);
-    return e;
-}
-
-static int build_backward_graph(lenia_world *w) {
-    AdjWorkspace *ws = w->adj;
-    hipStream_t s = w->ctx->stream;
-    if (ws->graph) (void)hipGraphExecDestroy(ws->graph);
-    ws->graph = nullptr;
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(e, "backward graph capture");
-    hipError_t eb = enqueue_backward(w);
-    e = hipStreamEndCapture(s, &graph);
-    if (eb != hipSuccess) e = eb;
-    if (e == hipSuccess) e = hipGraphInstantiate(&ws->graph, graph, nullptr, nullptr, 0);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        ws->graph = nullptr;
-        return hip_fail(e, "backward graph");
-    }
+    if (e != hipSuccess) return hip_fail(e, "rollout forward");
+    ws->ckptValid = true;
+    ws->ckptVersion = w->version;
     return LENIA_OK;
 }
 
 extern "C" int lenia_rollout_backward(lenia_world *w, uint32_t T, uint32_t ckptEvery,
                                       const float *A0, const float *gAT, float *gA0,
                                       lenia_param_grads *grads, int onDevice) {
-    if (!w || !A0 || !gAT || T == 0)
+    if (!w || !gAT || T == 0)
         return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_backward: bad argument");
     int rc = check_differentiable(w);
     if (rc) return rc;
     lenia_ctx *ctx = w->ctx;
     (void)hipSetDevice(ctx->device);
-    const uint32_t c = ckptEvery ? std::min(ckptEvery, T) : (uint32_t)std::max(1.0, std::floor(std::sqrt((double)T)));
+    const uint32_t c = ckpt_spacing(T, ckptEvery);
+    if (!A0) {
+        const AdjWorkspace *ws0 = w->adj;
+        if (!ws0 || !ws0->ckptValid || ws0->T != (int)T || ws0->c != (int)c || ws0->ckptVersion != w->version)
+            return fail(LENIA_ERR_STATE, "lenia_rollout_backward: A0 is NULL but no forward rollout "
+                        "with T=%u, checkpoints every %u and the current parameters precedes it", T, c);
+    }
     rc = ws_prepare(w, T, c);
+    if (!rc) rc = ensure_graphs(w);
     if (rc) return rc;
     AdjWorkspace *ws = w->adj;
     const size_t bytes = sizeof(float) * ws->n;
     hipStream_t s = ctx->stream;
-    hipError_t e = hipMemcpyAsync(ws->ckpt, A0, bytes, kind_in(onDevice), s);
-    if (e == hipSuccess) e = hipMemcpyAsync(ws->g, gAT, bytes, kind_in(onDevice), s);
-    if (e != hipSuccess) return hip_fail(e, "rollout backward inputs");
-    if (!ws->graph || ws->graphShapes != w->shapeVersion) {
-        rc = build_backward_graph(w);
-        if (rc) return rc;
-        ws->graphShapes = w->shapeVersion;
+    hipError_t e = hipSuccess;
+    if (A0) {
+        ws->ckptValid = false;
+        e = hipMemcpyAsync(ws->ckpt, A0, bytes, kind_in(onDevice), s);
+        if (e == hipSuccess) e = hipGraphLaunch(ws->fwdGraph, s);
     }
-    e = hipGraphLaunch(ws->graph, s);
-    w->hostSteps += 2 * T;
+    if (e == hipSuccess) e = hipMemcpyAsync(ws->g, gAT, bytes, kind_in(onDevice), s);
+    if (e == hipSuccess) e = hipGraphLaunch(ws->bwdGraph, s);
+    w->hostSteps += (A0 ? 2 : 1) * T;
     w->needsInit = false;
     w->specValid = true;
     if (e == hipSuccess && gA0) e = hipMemcpyAsync(gA0, ws->g, bytes, kind_out(onDevice), s);
@@ -368,6 +449,10 @@ extern "C" int lenia_rollout_backward(lenia_world *w, uint32_t T, uint32_t ckptE
     if (e == hipSuccess) e = hipMemcpyAsync(acc, ws->acc, sizeof(acc), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "rollout backward");
+    if (A0) {
+        ws->ckptValid = true;
+        ws->ckptVersion = w->version;
+    }
     if (grads) {
         std::memset(grads, 0, sizeof(*grads));
         grads->dDt = acc[0];

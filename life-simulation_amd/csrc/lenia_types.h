@@ -82,6 +82,10 @@ struct FftWorldDev {
     float *G;          // per-kernel column-pass output: [K][Wh][H] complex
     const float *S;    // per-kernel real spectra / (W H): [K][Wh][H]
     StepClock *clock;
+    // optional copies of a step's intermediates for the reverse pass (null
+    // normally): the state before the step (H x W), the potentials u_k
+    // ([K][H][W]) and the 2D spectrum of the state ([Wh][H] complex)
+    float *saveA, *saveU, *saveAhat;
 };
 
 // device descriptor of one direct-path world

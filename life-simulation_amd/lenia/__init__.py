@@ -69,6 +69,7 @@ class World:
         self.desc = dict(desc)
         self.W, self.H = int(desc["width"]), int(desc["height"])
         self._h = C.c_void_p()
+        self._rollouts = 0  # forward sweeps run (each replaces the kept checkpoints)
         d = _abi.world_desc(desc)
         check(_abi.lib().lenia_world_create(ctx.handle, C.byref(d), C.byref(self._h)))
 
@@ -118,20 +119,31 @@ class World:
         check(_abi.lib().lenia_world_pass_times(self._h, n, out, 4, C.byref(npass)))
         return [out[i] for i in range(npass.value)]
 
-    def rollout(self, A0, T: int) -> np.ndarray:
-        """T steps from A0 (host arrays); returns A_T"""
+    def rollout(self, A0, T: int, ckpt_every: int = 0) -> np.ndarray:
+        """T steps from A0 (host arrays); returns A_T and keeps the checkpoints
+        for rollout_grad(None, ...)"""
         a = np.ascontiguousarray(A0, dtype=np.float32)
         out = np.empty_like(a)
-        check(_abi.lib().lenia_rollout_forward(self._h, T, a.ctypes.data, out.ctypes.data, 0))
+        self._rollouts += 1
+        check(_abi.lib().lenia_rollout_forward(self._h, T, ckpt_every, a.ctypes.data,
+                                               out.ctypes.data, 0))
         return out
 
+    def version_token(self) -> int:
+        """changes whenever a forward sweep replaces the kept checkpoints"""
+        return self._rollouts
+
     def rollout_grad(self, A0, T: int, gAT, ckpt_every: int = 0) -> dict:
-        """reverse pass of rollout(A0, T) for dL/dA_T = gAT (host arrays)"""
-        a = np.ascontiguousarray(A0, dtype=np.float32)
+        """reverse pass of rollout(A0, T) for dL/dA_T = gAT (host arrays);
+        A0 None reuses the checkpoints of the last rollout()"""
         g = np.ascontiguousarray(gAT, dtype=np.float32)
-        gA0 = np.empty_like(a)
+        a = None if A0 is None else np.ascontiguousarray(A0, dtype=np.float32)
+        gA0 = np.empty_like(g)
         pg = _abi.ParamGrads()
-        check(_abi.lib().lenia_rollout_backward(self._h, T, ckpt_every, a.ctypes.data,
+        if a is not None:
+            self._rollouts += 1
+        check(_abi.lib().lenia_rollout_backward(self._h, T, ckpt_every,
+                                                None if a is None else a.ctypes.data,
                                                 g.ctypes.data, gA0.ctypes.data, C.byref(pg), 0))
         return grads_dict(pg, len(self.desc["kernels"]), gA0)
 

@@ -146,7 +146,7 @@ def lib():
         "lenia_batch_create": (i32, [vp, C.POINTER(vp), u32, C.POINTER(vp)]),
         "lenia_batch_destroy": (i32, [vp]),
         "lenia_batch_step": (i32, [vp, u32, u32]),
-        "lenia_rollout_forward": (i32, [vp, u32, vp, vp, i32]),
+        "lenia_rollout_forward": (i32, [vp, u32, u32, vp, vp, i32]),
         "lenia_world_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),

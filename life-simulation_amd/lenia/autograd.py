@@ -56,9 +56,12 @@ class LeniaRollout(torch.autograd.Function):
         A0c = A0.detach().contiguous()
         AT = torch.empty_like(A0c)
         _sync_torch()
-        check(_abi.lib().lenia_rollout_forward(world.handle, T, A0c.data_ptr(), AT.data_ptr(), 1))
+        world._rollouts += 1
+        check(_abi.lib().lenia_rollout_forward(world.handle, T, ckpt_every, A0c.data_ptr(),
+                                               AT.data_ptr(), 1))
         ctx.save_for_backward(A0c, mu, sigma, weight, betas, dt)
         ctx.world, ctx.T, ctx.ckpt = world, T, ckpt_every
+        ctx.world_version = world.version_token()
         return AT
 
     @staticmethod
@@ -70,7 +73,13 @@ class LeniaRollout(torch.autograd.Function):
         gA0 = torch.empty_like(A0c)
         pg = _abi.ParamGrads()
         _sync_torch()
-        check(_abi.lib().lenia_rollout_backward(world.handle, ctx.T, ctx.ckpt, A0c.data_ptr(),
+        # the checkpoints of this forward are reused when nothing changed since
+        # (same world, T, spacing and parameters); else the sweep reruns from A0
+        ws_ok = ctx.world_version == world.version_token()
+        if not ws_ok:
+            world._rollouts += 1
+        check(_abi.lib().lenia_rollout_backward(world.handle, ctx.T, ctx.ckpt,
+                                                None if ws_ok else A0c.data_ptr(),
                                                 g.data_ptr(), gA0.data_ptr(), C.byref(pg), 1))
         K = mu.shape[0]
         dev = mu.device

@@ -115,6 +115,33 @@ def test_gradient_checkpoint_spacing_and_repeat_bit_identical(ctx):
     w.close()
 
 
+def test_gradient_reuses_forward_checkpoints(ctx):
+    """rollout() keeps its checkpoints; rollout_grad(None, ...) reuses them and
+    gives the same bits as a reverse pass that reruns the sweep from A0; a
+    parameter change in between makes the reuse an error"""
+    import lenia
+
+    w = lenia.World(ctx, grad_world())
+    A0, gT = small_inputs(6)
+    T, c = 9, 3
+    full = w.rollout_grad(A0, T, gT, ckpt_every=c)
+    w.rollout(A0, T, ckpt_every=c)
+    reuse = w.rollout_grad(None, T, gT, ckpt_every=c)
+    assert np.array_equal(full["A0"], reuse["A0"])
+    for key in ("mu", "sigma", "weight", "betas"):
+        assert np.array_equal(full[key], reuse[key]), key
+    assert full["dt"] == reuse["dt"]
+    with pytest.raises(lenia.LeniaError) as ei:
+        w.rollout_grad(None, T + 1, gT, ckpt_every=c)
+    assert ei.value.code == -5
+    w.rollout(A0, T, ckpt_every=c)
+    w.set_params(grad_world(dt=0.12))
+    with pytest.raises(lenia.LeniaError) as ei:
+        w.rollout_grad(None, T, gT, ckpt_every=c)
+    assert ei.value.code == -5
+    w.close()
+
+
 def seeded_state(ctx, desc, seed, steps=0):
     """the world's own seeded init (full-frame noise), then `steps` steps"""
     import lenia
