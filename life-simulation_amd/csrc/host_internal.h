@@ -54,7 +54,6 @@ struct lenia_world {
     float *A0 = nullptr, *A1 = nullptr;
     float *scratchOut = nullptr;
     lenia::StepClock *clock = nullptr;
-    uint32_t *done = nullptr;
     void *devDesc = nullptr;
     lenia::FftWorldDev fdev{};
     lenia::DirectWorldDev ddev{};
@@ -92,7 +91,9 @@ size_t lds_cols(int H, int C, int K, bool kpar);
 bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
 bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
                 int &ntOut, int &TOut, bool &kparOut);
-hipError_t enqueue_step(lenia_world *w, const hipEvent_t *ev);
+hipError_t enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t off);
+hipError_t world_advance_clock(lenia_world *w, uint32_t by);
+hipError_t enqueue_steps(lenia_world *w, uint32_t n);
 int world_ensure_spectrum(lenia_world *w);
 int world_apply_init(lenia_world *w);
 

@@ -75,7 +75,7 @@ __device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const
 template <int MODE>
 __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles,
-                                                uint32_t *done, int nWorlds, int kSel,
+                                                uint32_t stepOff, int kSel,
                                                 float *__restrict__ out) {
     extern __shared__ float tileS[];
     const int tid = threadIdx.x;
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const int x0 = (tile.y % wd.tilesX) * kDTX;
         const int y0 = (tile.y / wd.tilesX) * kDTY;
         const int rows = kDTY + 2 * h, cols = kDTX + np, pitch = cols + 4;
-        const StepClock ck = read_clock(wd.clock);
+        const StepClock ck = clock_at(wd.clock, stepOff);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
         // halo rows y0-h .. y0+15+h, columns x0-h .. x0-h+cols-1 (toroidal;
@@ -176,18 +176,17 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         }
         __syncthreads();
     }
-    if (MODE == 0) arrive_step(done, worlds, nWorlds);
 }
 
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
-                         int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                         int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
                          hipStream_t s) {
     if (mode == 0)
-        launch_timed(k_direct<0>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
-                     nWorlds, kSel, out);
+        launch_timed(k_direct<0>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, stepOff,
+                     kSel, out);
     else
-        launch_timed(k_direct<1>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, done,
-                     nWorlds, kSel, out);
+        launch_timed(k_direct<1>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, stepOff,
+                     kSel, out);
     return hipGetLastError();
 }
 

@@ -141,7 +141,7 @@ __device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, c
 template <int MODE, int T, bool EXTRAS, bool KPAR>
 __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles,
-                                                uint32_t *done, int nWorlds, int kSel,
+                                                uint32_t stepOff, int kSel,
                                                 float *__restrict__ out) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
             continue;
         }
         const GrowthParams &g = wd.g;
-        const StepClock ck = read_clock(wd.clock);
+        const StepClock ck = clock_at(wd.clock, stepOff);
         const uint32_t seed = clock_seed(ck, g.moduleSeed);
         const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
         const size_t gStride = (size_t)Wh * H2;
@@ -327,7 +327,6 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
         forward_rows<T>(Z, wd, tw, rp0, nt, tid);
         __syncthreads();
     }
-    if (MODE == 1) arrive_step(done, worlds, nWorlds);
 }
 
 // ---------------------------------------------------------------- column pass
@@ -519,41 +518,41 @@ __global__ void k_spec_cols(const double *__restrict__ Cw, int h, int H, int Wh,
 // ---------------------------------------------------------------- launchers
 template <int T, bool EX, bool KP>
 static void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                        int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                        int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
                         hipStream_t s) {
     if (mode == 0)
         launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, done, nWorlds, kSel, out);
+                     nTiles, stepOff, kSel, out);
     else if (mode == 1)
         launch_timed(k_fft_rows<1, T, EX, KP>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
-                     done, nWorlds, kSel, out);
+                     stepOff, kSel, out);
     else
         launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, done, nWorlds, kSel, out);
+                     nTiles, stepOff, kSel, out);
 }
 
 template <int T>
 static void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
-                          const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
-                          int nWorlds, int kSel, float *out, hipStream_t s) {
+                          const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
+                          int kSel, float *out, hipStream_t s) {
     if (extras) {
-        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
-        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     } else {
-        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
-        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s);
+        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     }
 }
 
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
-                           size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                           size_t lds, uint32_t stepOff, int kSel, float *out,
                            hipStream_t s) {
     switch (threads) {
-        case 64: rows_dispatch<64>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 128: rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 256: rows_dispatch<256>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 512: rows_dispatch<512>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 64: rows_dispatch<64>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 128: rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 256: rows_dispatch<256>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 512: rows_dispatch<512>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -61,6 +61,11 @@ __global__ void k_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t
     *c = v;
 }
 
+template <class WorldT>
+__global__ void k_advance_clocks(const WorldT *worlds, int n, uint32_t by) {
+    advance_clocks(worlds, n, by);
+}
+
 __global__ void k_fill(float *p, size_t n, float v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -84,6 +89,22 @@ hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t
                             uint32_t parity, hipStream_t s) {
     hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, s, c, frame, sub, substeps, parity);
     return hipGetLastError();
+}
+
+template <class WorldT>
+static hipError_t advance_launch(const WorldT *worlds, int n, uint32_t by, hipStream_t s) {
+    if (n <= 0 || by == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_advance_clocks<WorldT>, dim3(1), dim3(256), 0, s, worlds, n, by);
+    return hipGetLastError();
+}
+hipError_t launch_advance_clocks(const FftWorldDev *worlds, int n, uint32_t by, hipStream_t s) {
+    return advance_launch(worlds, n, by, s);
+}
+hipError_t launch_advance_clocks(const DirectWorldDev *worlds, int n, uint32_t by, hipStream_t s) {
+    return advance_launch(worlds, n, by, s);
+}
+hipError_t launch_advance_clocks(const MfmaWorldDev *worlds, int n, uint32_t by, hipStream_t s) {
+    return advance_launch(worlds, n, by, s);
 }
 
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s) {

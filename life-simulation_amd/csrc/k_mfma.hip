@@ -121,7 +121,7 @@ __device__ __forceinline__ void rows_mma(f32x16 (&total)[NB], const Frags<SPLIT,
 template <int MODE, bool SPLIT, int KS>
 __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ worlds,
                                               const int2 *__restrict__ tiles, int nTiles,
-                                              uint32_t *done, int nWorlds, int kSel,
+                                              uint32_t stepOff, int kSel,
                                               float *__restrict__ out) {
     constexpr int NB = blocks_y<SPLIT>();
     extern __shared__ unsigned short smemMf[];
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         unsigned short *tHi = smemMf;
         unsigned short *tLo = smemMf + plane;
         float *red = reinterpret_cast<float *>(smemMf + (SPLIT ? 2 : 1) * plane);
-        const StepClock ck = read_clock(wd.clock);
+        const StepClock ck = clock_at(wd.clock, stepOff);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
         // state tile: rows y0-h .. y0+32NB-1+h, columns x0-h .. x0-h+cols-1
@@ -253,42 +253,41 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         }
         __syncthreads();
     }
-    if (MODE == 0) arrive_step(done, worlds, nWorlds);
 }
 
 template <int KS>
 static void mfma_launch_ks(int mode, bool split, const MfmaWorldDev *worlds, const int2 *tiles,
-                           int nTiles, int grid, size_t lds, uint32_t *done, int nWorlds, int kSel,
+                           int nTiles, int grid, size_t lds, uint32_t stepOff, int kSel,
                            float *out, hipStream_t s) {
     if (mode == 0) {
         if (split)
             launch_timed(k_mfma<0, true, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
-                         done, nWorlds, kSel, out);
+                         stepOff, kSel, out);
         else
             launch_timed(k_mfma<0, false, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
-                         done, nWorlds, kSel, out);
+                         stepOff, kSel, out);
     } else {
         if (split)
             launch_timed(k_mfma<1, true, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
-                         done, nWorlds, kSel, out);
+                         stepOff, kSel, out);
         else
             launch_timed(k_mfma<1, false, KS>, dim3(grid), dim3(kMT), lds, s, worlds, tiles, nTiles,
-                         done, nWorlds, kSel, out);
+                         stepOff, kSel, out);
     }
 }
 
 // ksteps = ceil((32 + 2h) / 16): 2 (h = 0) .. 8 (h <= 48)
 hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,
-                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
-                       int nWorlds, int kSel, float *out, hipStream_t s) {
+                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
+                       int kSel, float *out, hipStream_t s) {
     switch (ksteps) {
-        case 2: mfma_launch_ks<2>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 3: mfma_launch_ks<3>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 4: mfma_launch_ks<4>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 5: mfma_launch_ks<5>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 6: mfma_launch_ks<6>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 7: mfma_launch_ks<7>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
-        case 8: mfma_launch_ks<8>(mode, split, worlds, tiles, nTiles, grid, lds, done, nWorlds, kSel, out, s); break;
+        case 2: mfma_launch_ks<2>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 3: mfma_launch_ks<3>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 4: mfma_launch_ks<4>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 5: mfma_launch_ks<5>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 6: mfma_launch_ks<6>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 7: mfma_launch_ks<7>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 8: mfma_launch_ks<8>(mode, split, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -29,11 +29,12 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_
 }
 
 // FFT path (k_fft.hip).  mode 0: forward row FFT of the state; 1: full step
-// row pass; 2: potential of kernel kSel into `out`.
+// row pass; 2: potential of kernel kSel into `out`.  The step kernels run the
+// step `stepOff` steps after each world's stored clock (mode 1 / mode 0 of
+// the direct forms).
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
-                           size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
-                           hipStream_t s);
+                           size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s);
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
                            int nTiles, int grid, size_t lds, hipStream_t s);
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s);
@@ -44,7 +45,7 @@ int fft_max_values();
 
 // direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
-                         int grid, size_t lds, uint32_t *done, int nWorlds, int kSel, float *out,
+                         int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
                          hipStream_t s);
 void direct_tile_shape(int *tx, int *ty);
 int direct_npad(int h);
@@ -52,8 +53,8 @@ size_t direct_lds_bytes(int h);
 
 // MFMA direct forms (k_mfma.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,
-                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t *done,
-                       int nWorlds, int kSel, float *out, hipStream_t s);
+                       const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
+                       int kSel, float *out, hipStream_t s);
 void mfma_tile_shape(bool split, int *tx, int *ty);
 int mfma_max_ksteps();
 size_t mfma_lds_bytes(int h, bool split);
@@ -79,6 +80,11 @@ hipError_t launch_init_noise(float *A, int W, int H, uint32_t seed, float covera
                              hipStream_t s);
 hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t substeps,
                             uint32_t parity, hipStream_t s);
+// move the stored clocks of n worlds `by` steps on (after the step kernels
+// that read them with offsets 0 .. by-1)
+hipError_t launch_advance_clocks(const FftWorldDev *worlds, int n, uint32_t by, hipStream_t s);
+hipError_t launch_advance_clocks(const DirectWorldDev *worlds, int n, uint32_t by, hipStream_t s);
+hipError_t launch_advance_clocks(const MfmaWorldDev *worlds, int n, uint32_t by, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 
 }  // namespace lenia

@@ -266,24 +266,27 @@ static hipError_t load_state(lenia_world *w, const float *src, hipStream_t s) {
                                   hipMemcpyDeviceToDevice, s);
     if (e == hipSuccess)
         e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles,
-                            w->nRowTiles, w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, s);
+                            w->nRowTiles, w->nRowTiles, w->ldsRows, 0, 0, nullptr, s);
     return e;
 }
 
-// forward sweep from ckpt[0]: checkpoints at t = c, 2c, ...; the world ends at A_T
+// forward sweep from ckpt[0]: checkpoints at t = c, 2c, ...; the world ends at
+// A_T with its clock T steps on
 static hipError_t enqueue_forward(lenia_world *w) {
     AdjWorkspace *ws = w->adj;
     hipStream_t s = w->ctx->stream;
     const size_t n = ws->n;
     const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
     hipError_t e = load_state(w, ws->ckpt, s);
+    uint32_t t = 0;
     for (uint32_t k = 1; k < nck && e == hipSuccess; ++k) {
-        for (uint32_t i = 0; i < c && e == hipSuccess; ++i) e = enqueue_step(w, nullptr);
+        for (uint32_t i = 0; i < c && e == hipSuccess; ++i) e = enqueue_step(w, nullptr, t++);
         if (e == hipSuccess)
             e = hipMemcpyAsync(ws->ckpt + n * k, w->current_state(), sizeof(float) * n,
                                hipMemcpyDeviceToDevice, s);
     }
-    for (uint32_t t = (nck - 1) * c; t < T && e == hipSuccess; ++t) e = enqueue_step(w, nullptr);
+    while (t < T && e == hipSuccess) e = enqueue_step(w, nullptr, t++);
+    if (e == hipSuccess) e = world_advance_clock(w, T);
     return e;
 }
 
@@ -299,7 +302,7 @@ static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
                             w->ldsCols, s);
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, false, w->rowKpar, recs + j, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, s);
+                                w->nRowTiles, w->ldsRows, 0, 0, nullptr, s);
     }
     return e;
 }

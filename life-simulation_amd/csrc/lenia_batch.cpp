@@ -31,7 +31,6 @@ struct lenia_batch {
     int2 *rowTiles = nullptr, *colTiles = nullptr;
     int nRowTiles = 0, nColTiles = 0;
     size_t ldsRows = 0, ldsCols = 0;
-    uint32_t *done = nullptr;
     hipGraphExec_t graph = nullptr;
     uint32_t graphSteps = 0;
     std::vector<uint32_t> versions;  // world descriptor versions the table holds
@@ -51,11 +50,7 @@ static void batch_release_tables(lenia_batch *b) {
     b->ldsRows = b->ldsCols = 0;
 }
 
-static void batch_release(lenia_batch *b) {
-    batch_release_tables(b);
-    dev_free(b->ctx, b->done);
-    b->done = nullptr;
-}
+static void batch_release(lenia_batch *b) { batch_release_tables(b); }
 
 static int batch_setup_fft(lenia_batch *b);
 static int batch_setup_direct(lenia_batch *b);
@@ -203,13 +198,7 @@ extern "C" int lenia_batch_create(lenia_ctx *ctx, lenia_world *const *worlds, ui
     b->worlds.assign(worlds, worlds + n);
     b->isFft = worlds[0]->isFft;
     for (lenia_world *w : b->worlds) b->extras = b->extras || w->extras();
-    int rc = dev_alloc(ctx, (void **)&b->done, sizeof(uint32_t) * 4);
-    if (!rc) {
-        hipError_t e = hipMemsetAsync(b->done, 0, sizeof(uint32_t) * 4, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "batch counter");
-    }
-    if (!rc) rc = batch_setup(b);
+    int rc = batch_setup(b);
     if (rc) {
         batch_release(b);
         delete b;
@@ -228,8 +217,9 @@ extern "C" int lenia_batch_destroy(lenia_batch *b) {
     return LENIA_OK;
 }
 
-// one step of every world; with `ev`, kernel p records ev[2p] and ev[2p+1]
-static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev = nullptr) {
+// one step of every world, `off` steps after its stored clock; with `ev`,
+// kernel p records ev[2p] and ev[2p+1]
+static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint32_t off) {
     lenia_ctx *ctx = b->ctx;
     const int n = (int)b->worlds.size();
     hipError_t e;
@@ -240,14 +230,29 @@ static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev = null
         if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
             e = launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, (const FftWorldDev *)b->devWorlds,
-                                b->rowTiles, b->nRowTiles, b->nRowTiles, b->ldsRows, b->done, n, 0,
+                                b->rowTiles, b->nRowTiles, b->nRowTiles, b->ldsRows, off, 0,
                                 nullptr, ctx->stream);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                          b->nRowTiles, b->ldsRows, b->done, n, 0, nullptr, ctx->stream);
+                          b->nRowTiles, b->ldsRows, off, 0, nullptr, ctx->stream);
     }
     launch_events() = LaunchEvents();
+    return e;
+}
+
+// every world's stored clock moves `by` steps on
+static hipError_t batch_advance(lenia_batch *b, uint32_t by) {
+    const int n = (int)b->worlds.size();
+    if (b->isFft) return launch_advance_clocks((const FftWorldDev *)b->devWorlds, n, by, b->ctx->stream);
+    return launch_advance_clocks((const DirectWorldDev *)b->devWorlds, n, by, b->ctx->stream);
+}
+
+// n steps of every world (offsets 0 .. n-1), then the clocks move n steps on
+static hipError_t batch_enqueue_steps(lenia_batch *b, uint32_t n) {
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i) e = batch_enqueue_step(b, nullptr, i);
+    if (e == hipSuccess) e = batch_advance(b, n);
     return e;
 }
 
@@ -317,10 +322,12 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     auto time_now = [&](double ms[2]) -> int {
         int r = batch_setup(b);
         if (r) return r;
-        hipError_t et = hipSuccess;
-        for (int i = 0; i < 2 && et == hipSuccess; ++i) et = batch_enqueue_step(b);
+        hipError_t et = batch_enqueue_steps(b, 2);
         if (et == hipSuccess)
-            et = time_steps(ctx, nSteps, 2, [&](const hipEvent_t *ev) { return batch_enqueue_step(b, ev); }, ms);
+            et = time_steps(ctx, nSteps, 2, [&](const hipEvent_t *ev) {
+                hipError_t es = batch_enqueue_step(b, ev, 0);
+                return es == hipSuccess ? batch_advance(b, 1) : es;
+            }, ms);
         return et == hipSuccess ? LENIA_OK : hip_fail(et, "batch autotune timing");
     };
     double best[2] = {1e30, 1e30};
@@ -398,8 +405,7 @@ extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphS
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
             if (e != hipSuccess) return hip_fail(e, "batch graph capture");
-            hipError_t el = hipSuccess;
-            for (uint32_t i = 0; i < graphSteps && el == hipSuccess; ++i) el = batch_enqueue_step(b);
+            hipError_t el = batch_enqueue_steps(b, graphSteps);
             e = hipStreamEndCapture(ctx->stream, &g);
             if (el != hipSuccess) e = el;
             if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
@@ -415,10 +421,8 @@ extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphS
             if (e != hipSuccess) return hip_fail(e, "batch graph launch");
         }
     } else {
-        for (uint32_t i = 0; i < nSteps; ++i) {
-            hipError_t e = batch_enqueue_step(b);
-            if (e != hipSuccess) return hip_fail(e, "batch step launch");
-        }
+        hipError_t e = batch_enqueue_steps(b, nSteps);
+        if (e != hipSuccess) return hip_fail(e, "batch step launch");
     }
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->timed = true;

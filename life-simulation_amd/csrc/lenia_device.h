@@ -35,8 +35,22 @@ __device__ __forceinline__ float2 gld2(const LENIA_GLOBAL float2 *p) {
 __device__ __forceinline__ void gst2(LENIA_GLOBAL float2 *p, float2 a) {
     *(LENIA_GLOBAL lv2 *)p = lv2{a.x, a.y};
 }
-// frame / sub of a world's step clock
-__device__ __forceinline__ StepClock read_clock(const StepClock *c) {
+// The clock `off` steps after the stored one (sub < substeps): what `off`
+// single-step advances (sub + 1, wrapping into the next frame) give.
+__device__ __forceinline__ StepClock clock_after(StepClock v, uint32_t off) {
+    const uint32_t s = v.sub + off, n = v.substeps ? v.substeps : 1u;
+    v.frame += s / n;
+    v.sub = s % n;
+    v.parity ^= off & 1u;
+    v.steps += off;
+    return v;
+}
+
+// The clock of the step `off` steps after a world's stored clock.  Steps
+// launched together read the same stored clock with offsets 0, 1, 2, ... and
+// one small kernel moves it on after the last of them, so no step kernel
+// writes the clock that its own blocks read.
+__device__ __forceinline__ StepClock clock_at(const StepClock *c, uint32_t off) {
     const LENIA_GLOBAL StepClock *g = gp(c);
     StepClock v;
     v.frame = g->frame;
@@ -45,7 +59,7 @@ __device__ __forceinline__ StepClock read_clock(const StepClock *c) {
     v.parity = g->parity;
     v.steps = g->steps;
     v.pad[0] = v.pad[1] = v.pad[2] = 0u;
-    return v;
+    return clock_after(v, off);
 }
 
 // pcg_hash (Shaders/Common/Random.metal:5-9)
@@ -153,38 +167,18 @@ __device__ __forceinline__ uint32_t clock_seed(const StepClock &c, uint32_t modu
     return pcg_hash(moduleSeed ^ (c.frame * 197u + c.sub));
 }
 
-// Advance the step clock of every world of a launch once all its blocks are
-// done.  Called by every block at the end of the last kernel of a step: the
-// block that arrives last (counter) moves the clocks on with the 64 lanes of
-// its first wave and re-arms the counter.  The next kernel in stream order
-// sees the new values.
+// Move every world's stored clock `by` steps on: one thread per world, run by
+// a one-block kernel after the last step kernel of a launch sequence.
 template <class WorldT>
-__device__ __forceinline__ void arrive_step(uint32_t *done, const WorldT *worlds, int nWorlds) {
-    __syncthreads();
-    if (threadIdx.x >= 64) return;
-    const int lane = threadIdx.x;
-    int last = 0;
-    if (lane == 0) {
-        __threadfence();
-        last = atomicAdd(done, 1u) == gridDim.x - 1 ? 1 : 0;
-    }
-    last = __shfl(last, 0, 64);
-    if (!last) return;
-    __threadfence();
-    for (int i = lane; i < nWorlds; i += 64) {
+__device__ __forceinline__ void advance_clocks(const WorldT *worlds, int nWorlds, uint32_t by) {
+    for (int i = threadIdx.x; i < nWorlds; i += blockDim.x) {
         LENIA_GLOBAL StepClock *cp = gp(worlds[i].clock);
-        uint32_t sub = cp->sub + 1u, frame = cp->frame;
-        if (sub >= cp->substeps) {
-            sub = 0u;
-            frame += 1u;
-        }
-        cp->sub = sub;
-        cp->frame = frame;
-        cp->parity ^= 1u;
-        cp->steps += 1u;
+        StepClock v = clock_at(worlds[i].clock, by);
+        cp->frame = v.frame;
+        cp->sub = v.sub;
+        cp->parity = v.parity;
+        cp->steps = v.steps;
     }
-    __threadfence();
-    if (lane == 0) atomicExch(done, 0u);
 }
 
 }  // namespace lenia

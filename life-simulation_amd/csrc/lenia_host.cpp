@@ -683,7 +683,7 @@ static void world_release(lenia_world *w) {
     world_drop_graph(w);
     world_free_adjoint(w);
     void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF, w->toepHi, w->toepMid, w->toepLo,
-                    w->clock, w->done, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
+                    w->clock, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
     for (void *p : ptrs) dev_free(ctx, p);
 }
 
@@ -741,7 +741,6 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
     rc = dev_alloc(ctx, (void **)&w->A0, cells * sizeof(float));
     if (!rc && !w->isFft) rc = dev_alloc(ctx, (void **)&w->A1, cells * sizeof(float));
     if (!rc) rc = dev_alloc(ctx, (void **)&w->clock, sizeof(StepClock));
-    if (!rc) rc = dev_alloc(ctx, (void **)&w->done, sizeof(uint32_t) * 4);
     if (!rc)
         rc = dev_alloc(ctx, (void **)&w->devDesc,
                        std::max({sizeof(FftWorldDev), sizeof(DirectWorldDev), sizeof(MfmaWorldDev)}));
@@ -770,8 +769,7 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         delete w;
         return rc;
     }
-    hipError_t e = hipMemsetAsync(w->done, 0, sizeof(uint32_t) * 4, ctx->stream);
-    if (e == hipSuccess) e = launch_fill(w->A0, cells, 0.0f, ctx->stream);
+    hipError_t e = launch_fill(w->A0, cells, 0.0f, ctx->stream);
     if (e == hipSuccess && w->A1) e = launch_fill(w->A1, cells, 0.0f, ctx->stream);
     if (e == hipSuccess) e = launch_set_clock(w->clock, 0, 0, 1, 0, ctx->stream);
     if (e == hipSuccess && w->isFft) {
@@ -855,7 +853,7 @@ int lenia::world_ensure_spectrum(lenia_world *w) {
     if (!w->isFft || w->specValid) return LENIA_OK;
     lenia_ctx *ctx = w->ctx;
     hipError_t e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                   w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+                                   w->nRowTiles, w->ldsRows, 0, 0, nullptr, ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "forward row pass");
     w->specValid = true;
     return LENIA_OK;
@@ -864,9 +862,9 @@ int lenia::world_ensure_spectrum(lenia_world *w) {
 static thread_local LaunchEvents tl_launch_events;
 LaunchEvents &lenia::launch_events() { return tl_launch_events; }
 
-// one step's kernels; with `ev`, kernel p of the step records ev[2p] and
-// ev[2p+1] around itself
-hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev) {
+// one step's kernels, the step `off` steps after the stored clock; with `ev`,
+// kernel p of the step records ev[2p] and ev[2p+1] around itself
+hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t off) {
     lenia_ctx *ctx = w->ctx;
     hipError_t e;
     if (w->isFft) {
@@ -877,17 +875,32 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev) {
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
                                 (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+                                w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
     } else if (w->isMfma) {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                        w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+                        w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, w->done, 1, 0, nullptr, ctx->stream);
+                          w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
     }
     launch_events() = LaunchEvents();
+    return e;
+}
+
+hipError_t lenia::world_advance_clock(lenia_world *w, uint32_t by) {
+    hipStream_t s = w->ctx->stream;
+    if (w->isFft) return launch_advance_clocks((const FftWorldDev *)w->devDesc, 1, by, s);
+    if (w->isMfma) return launch_advance_clocks((const MfmaWorldDev *)w->devDesc, 1, by, s);
+    return launch_advance_clocks((const DirectWorldDev *)w->devDesc, 1, by, s);
+}
+
+// n steps (offsets 0 .. n-1), then the clock moves n steps on
+hipError_t lenia::enqueue_steps(lenia_world *w, uint32_t n) {
+    hipError_t e = hipSuccess;
+    for (uint32_t i = 0; i < n && e == hipSuccess; ++i) e = enqueue_step(w, nullptr, i);
+    if (e == hipSuccess) e = world_advance_clock(w, n);
     return e;
 }
 
@@ -896,9 +909,10 @@ void lenia::world_drop_graph(lenia_world *w) {
     w->graphExec = nullptr;
 }
 
-// Steps are replayed from a hipGraph of kGraphSteps steps, captured once per
-// world: kernels read their parameters and step clock from device memory, so
-// one graph serves every later replay.  Leftover steps launch directly.
+// Steps are replayed from a hipGraph of kGraphSteps steps and one clock
+// advance, captured once per world: kernels read their parameters and the
+// stored clock from device memory, so one graph serves every later replay.
+// Leftover steps launch directly.
 static constexpr uint32_t kGraphSteps = 32;
 
 static int world_build_graph(lenia_world *w) {
@@ -907,8 +921,7 @@ static int world_build_graph(lenia_world *w) {
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "graph capture");
-    hipError_t el = hipSuccess;
-    for (uint32_t i = 0; i < kGraphSteps && el == hipSuccess; ++i) el = enqueue_step(w, nullptr);
+    hipError_t el = enqueue_steps(w, kGraphSteps);
     e = hipStreamEndCapture(ctx->stream, &graph);
     if (el != hipSuccess) e = el;
     if (e == hipSuccess) e = hipGraphInstantiate(&w->graphExec, graph, nullptr, nullptr, 0);
@@ -937,10 +950,10 @@ int lenia::world_launch_steps(lenia_world *w, uint32_t n) {
         }
         n %= kGraphSteps;
     }
-    for (uint32_t i = 0; i < n; ++i) {
-        hipError_t e = enqueue_step(w, nullptr);
+    if (n) {
+        hipError_t e = enqueue_steps(w, n);
         if (e != hipSuccess) return hip_fail(e, "step launch");
-        w->hostSteps += 1;
+        w->hostSteps += n;
     }
     return LENIA_OK;
 }
@@ -1077,14 +1090,14 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
                             w->ldsCols, ctx->stream);
         if (e == hipSuccess)
             e = launch_fft_rows(2, w->rowT, false, true, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut,
+                                w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut,
                                 ctx->stream);
     } else if (w->isMfma) {
         e = launch_mfma(1, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                        w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut, ctx->stream);
+                        w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
     } else {
         e = launch_direct(1, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, w->done, 1, (int)k, w->scratchOut, ctx->stream);
+                          w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "potential launch");
     HIP_TRY(hipMemcpyAsync(host, w->scratchOut, cells * sizeof(float), hipMemcpyDeviceToHost,
@@ -1138,7 +1151,8 @@ extern "C" int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *o
     int rc = world_ensure_spectrum(w);
     if (rc) return rc;
     hipError_t e = time_steps(ctx, nSteps, np, [&](const hipEvent_t *ev) {
-        hipError_t es = enqueue_step(w, ev);
+        hipError_t es = enqueue_step(w, ev, 0);
+        if (es == hipSuccess) es = world_advance_clock(w, 1);
         w->hostSteps += 1;
         w->frameNext += 1;
         return es;

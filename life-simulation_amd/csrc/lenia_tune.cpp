@@ -38,11 +38,12 @@ std::vector<Shape> candidates(const FftPlan &p, bool rows, int count, int K) {
 int time_shape(lenia_world *w, uint32_t nSteps, double ms[2]) {
     int rc = world_upload_desc(w);
     if (rc) return rc;
-    for (int i = 0; i < 2; ++i) {
-        hipError_t e = enqueue_step(w, nullptr);
-        if (e != hipSuccess) return hip_fail(e, "autotune warm-up");
-    }
-    hipError_t e = time_steps(w->ctx, nSteps, 2, [&](const hipEvent_t *ev) { return enqueue_step(w, ev); }, ms);
+    hipError_t e = enqueue_steps(w, 2);
+    if (e != hipSuccess) return hip_fail(e, "autotune warm-up");
+    e = time_steps(w->ctx, nSteps, 2, [&](const hipEvent_t *ev) {
+        hipError_t es = enqueue_step(w, ev, 0);
+        return es == hipSuccess ? world_advance_clock(w, 1) : es;
+    }, ms);
     return e == hipSuccess ? LENIA_OK : hip_fail(e, "autotune timing");
 }
 

@@ -29,8 +29,8 @@ struct GrowthParams {
     uint32_t moduleSeed;  // seed_ = deriveSeed(sceneSeed, 'LEN1')
 };
 
-// frame / substep position of a world, advanced on the device by the last
-// kernel of every step
+// frame / substep position of a world.  A sequence of steps reads it with
+// per-step offsets; a one-block kernel moves it on after the sequence.
 struct StepClock {
     uint32_t frame;
     uint32_t sub;

@@ -280,6 +280,46 @@ def test_determinism(ctx, conv):
     assert outs[0] == outs[1]
 
 
+@pytest.mark.parametrize("conv", [FFT, DIRECT])
+def test_extras_substep_clock_matches_oracle(ctx, conv):
+    """Each substep of a frame and each later step uses its own step seed
+    pcg(seed_ ^ (frame*197 + sub)): frame(29, 3) runs subs 0..2 of frame 29,
+    then step(37) continues at frame 30 (through one captured 32-step
+    sequence and 5 single launches)."""
+    desc = world(80, 48, [ring(6, [1], .15, .017, 1.0), ring(5, [1, .6], .3, .035, .6)], conv,
+                 growthMode=1, dt=0.35, radius=6, noiseAmount=0.5, muJitter=0.8,
+                 injectAmount=0.4, injectCount=3)
+    A = rand_state(80, 48, seed=5)
+    m = O.module_seed(4242)
+    with make(ctx, desc) as seq, make(ctx, desc) as one:
+        for w in (seq, one):
+            w.reset(4242)
+            w.write(A)
+            w.frame(29, 3)
+        got3 = one.read()
+        want = A.astype(np.float64)
+        for s in range(3):
+            want = O.step(desc, want, use_fft=False, seed=O.step_seed(m, 29, s), frame=29)
+        mx3, _ = err(got3, want)
+        # one step per call, each checked from the GPU's own previous state (a
+        # wrong seed moves the noise term by ~noiseAmount * 0.03 >> 1e-5)
+        worst = 0.0
+        for i in range(37):
+            fr, sub = 30 + i // 3, i % 3
+            prev = one.read().astype(np.float64)
+            one.step(1)
+            want = O.step(desc, prev, use_fft=False, seed=O.step_seed(m, fr, sub), frame=fr)
+            worst = max(worst, err(one.read(), want)[0])
+        # the same 37 steps in one call: the captured sequence gives the same bytes
+        seq.step(37)
+        same = seq.read().tobytes() == one.read().tobytes()
+    print(f"conv={conv}: frame(29,3) max-abs {mx3:.2e}; single steps worst {worst:.2e}; "
+          f"one call == single calls: {same}")
+    assert mx3 <= 3 * STEP_TOL
+    assert worst <= STEP_TOL
+    assert same
+
+
 def test_frame_semantics_match_oracle(ctx):
     """init happens in the first frame, followed by `substeps` steps"""
     d = config4(conv=FFT)
