@@ -290,6 +290,74 @@ __device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt
     }
 }
 
+// The same Stockham stage with the plan constants known at compile time
+// (power-of-two radices and lengths): index arithmetic folds to shifts and
+// masks, the stage tables need no loads and there is no radix switch.  The
+// butterflies, twiddles and stores are those of fft_stage, so the bits are too.
+template <int R, bool INV, int MAXV, int T, int N, int NS, int TWOFF>
+__device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *tw, int tid) {
+    constexpr int NR = N / R;
+    constexpr int MB = MAXV / R;
+    static_assert((NR & (NR - 1)) == 0 && (NS & (NS - 1)) == 0, "power-of-two plans only");
+    if constexpr (MB > 0) {
+        const int nb = nt * NR;
+        float2 v[MB][R];
+        float2 w[MB][R - 1];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const int g = tid + i * T;
+            if (g < nb) {
+                const int b = g / NR, j = g % NR, k = j % NS;
+                const int base = b * N + j;
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[i][r] = buf[sw(base + r * NR)];
+                const float2 *t = tw + TWOFF + k * (R - 1);
+#pragma unroll
+                for (int r = 0; r < R - 1; ++r) w[i][r] = t[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < MB; ++i) {
+            const int g = tid + i * T;
+            if (g < nb) {
+                const int b = g / NR, j = g % NR, k = j % NS;
+                if (k != 0) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[i][r] = ctw<INV>(v[i][r], w[i][r - 1]);
+                }
+                Dft<R, INV>::run(v[i]);
+                const int base = b * N + (j - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) buf[sw(base + r * NS)] = v[i][r];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool INV, int MAXV, int T, int N, int NS, int TWOFF, int R, int... REST>
+__device__ __forceinline__ void fft_fixed(float2 *buf, int nt, const float2 *tw, int tid) {
+    fft_stage_c<R, INV, MAXV, T, N, NS, TWOFF>(buf, nt, tw, tid);
+    if constexpr (sizeof...(REST) > 0)
+        fft_fixed<INV, MAXV, T, N, NS * R, TWOFF + NS * (R - 1), REST...>(buf, nt, tw, tid);
+}
+
+// fft_lds through the compile-time kernel of the plan when it has one
+template <bool INV, int MAXV, int T>
+__device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
+                                        int tid) {
+    switch (plan.code) {
+        case kPlan64: fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, nt, tw, tid); break;
+        case kPlan128: fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, nt, tw, tid); break;
+        case kPlan256: fft_fixed<INV, MAXV, T, 256, 1, 0, 16, 16>(buf, nt, tw, tid); break;
+        case kPlan512: fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, nt, tw, tid); break;
+        case kPlan1024: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, nt, tw, tid); break;
+        case kPlan2048: fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, nt, tw, tid); break;
+        default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid); break;
+    }
+}
+
 // XCD-aware tile order: consecutive workgroup ids land on different XCDs
 // (round robin over 8), so give each XCD a contiguous run of tiles; tiles
 // that share cache lines (neighbouring row pairs) then share one L2.

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(T) void k_adj_edge(AdjArgs a, const float *G1, cons
     load_twiddles(tw, a.twW, W, tid, T);
     gather_row_pair<false>(Z, 0, G1, W, H2, rp, tid, T);
     __syncthreads();
-    fft_lds<true, kAV, T>(Z, a.planW, 1, tw, tid);
+    fft_run<true, kAV, T>(Z, a.planW, 1, tw, tid);
     const LENIA_GLOBAL float *ad = gp(gpb);
     LENIA_GLOBAL float *go = gp(gOut);
     for (int x = tid; x < W; x += T) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
             if (k < K) Z[sw(k * W + x)] = gu2[k];
     }
     __syncthreads();
-    fft_lds<false, kAV, T>(Z, a.planW, K, tw, tid);
+    fft_run<false, kAV, T>(Z, a.planW, K, tw, tid);
     for (int k = 0; k < K; ++k) split_row_pair(Z, k * W, FG + k * sStride * 2, W, H2, rp, tid, T);
     // fixed-order workgroup sums of the partials
     const int nv = 1 + 3 * K;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
         X[sw(e)] = gld2(gp(reinterpret_cast<const float2 *>(Fin)) + s * sStride + (size_t)c * H + ky);
     }
     __syncthreads();
-    fft_lds<false, kAV, T>(X, a.planH, K, tw, tid);
+    fft_run<false, kAV, T>(X, a.planH, K, tw, tid);
     float2 *Y = X + ((K * H + 31) & ~31);
     // shell partial sums: thread-strided over ky, then a fixed-order LDS tree
     const float wt = (c == 0 || c == Wh - 1) ? 1.0f : 2.0f;
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
         if (tid == 0) gp(accCols)[(size_t)c * 16 + i] += (double)red[0];
         __syncthreads();
     }
-    fft_lds<true, kAV, T>(Y, a.planH, 1, tw, tid);
+    fft_run<true, kAV, T>(Y, a.planH, 1, tw, tid);
     for (int e = tid; e < H; e += T) gst2(gp(reinterpret_cast<float2 *>(G)) + (size_t)c * H + e, Y[sw(e)]);
 }
 

@@ -118,7 +118,7 @@ __device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, c
                                              int rp0, int nt, int tid) {
     const int W = wd.W, H2 = wd.H / 2, P = wd.P;
     const int nm = P * wd.Wh;
-    fft_lds<false, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+    fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid);
     LENIA_GLOBAL float4 *F4 = gp(reinterpret_cast<float4 *>(wd.F)) + rp0;
     for (int m = tid; m < nm; m += T) {
         int kx = (int)fast_div((uint32_t)m, wd.mP), p = m - kx * P;
@@ -145,7 +145,8 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                                                 float *__restrict__ out) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
-    for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
+    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
+    if (ti < nTiles) do {
         const int2 tile = tiles[tile_of(ti, nTiles)];
         const FftWorldDev &wd = worlds[tile.x];
         const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
@@ -174,6 +175,9 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
             continue;
         }
         const GrowthParams &g = wd.g;
+        const GrowthRegs gr = growth_regs(g);
+        LENIA_GLOBAL float *const saveU = gp(wd.saveU);
+        LENIA_GLOBAL float *const saveA = gp(wd.saveA);
         const StepClock ck = clock_at(wd.clock, stepOff);
         const uint32_t seed = clock_seed(ck, g.moduleSeed);
         const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
             twc.store(tw, twg, W, tid);
             gather_store<T>(Z, gv, kn, W, nt, P, mP, nm, tid);
             __syncthreads();
-            fft_lds<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
+            fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
             if (MODE == 2) {
                 for (int q = tid; q < nq; q += T) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -223,19 +227,25 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                         ja = cell_jitter(g, x, ya, seed, ck.frame);
                         jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
                     }
+                    float2 z[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k < kn) z[k] = Z[sw(k * nq + q)];
                     float sa = 0.0f, sb = 0.0f;
-                    for (int k = 0; k < kn; ++k) {
-                        float2 z = Z[sw(k * nq + q)];
-                        sa += growth_term(g, k, z.x, ja);
-                        sb += growth_term(g, k, z.y, jb);
-                        if (wd.saveU) {
-                            LENIA_GLOBAL float *su = gp(wd.saveU) + (size_t)k * W * H + (size_t)ya * W + x;
-                            su[0] = z.x;
-                            su[W] = z.y;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (k < kn) {
+                            sa += growth_term_r(gr, k, z[k].x, ja);
+                            sb += growth_term_r(gr, k, z[k].y, jb);
+                            if (saveU) {
+                                LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
+                                su[0] = z[k].x;
+                                su[W] = z[k].y;
+                            }
                         }
                     }
-                    if (wd.saveA) {
-                        LENIA_GLOBAL float *sa0 = gp(wd.saveA) + (size_t)ya * W + x;
+                    if (saveA) {
+                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
                         sa0[0] = ar0[i];
                         sa0[W] = ar1[i];
                     }
@@ -243,8 +253,8 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                         ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
                         ar1[i] = growth_finish(g, ar1[i], sb, x, ya + 1, W, H, seed, ck.frame);
                     } else {
-                        ar0[i] = growth_plain(g, ar0[i], sa);
-                        ar1[i] = growth_plain(g, ar1[i], sb);
+                        ar0[i] = growth_plain_r(gr, ar0[i], sa);
+                        ar1[i] = growth_plain_r(gr, ar1[i], sb);
                     }
                 }
             }
@@ -273,6 +283,8 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                 if (k + 1 < (int)g.nKernels)
                     gather_issue<T>(gv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, tid);
                 __syncthreads();
+                // table-driven transform inside the runtime kernel loop: the
+                // compile-time one would keep its addresses live across it
                 fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
 #pragma unroll
                 for (int i = 0; i < kMaxV; ++i) {
@@ -286,10 +298,10 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                             ja = cell_jitter(g, x, ya, seed, ck.frame);
                             jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
                         }
-                        sa[i] += growth_term(g, k, z.x, ja);
-                        sb[i] += growth_term(g, k, z.y, jb);
-                        if (wd.saveU) {
-                            LENIA_GLOBAL float *su = gp(wd.saveU) + (size_t)k * W * H + (size_t)ya * W + x;
+                        sa[i] += growth_term_r(gr, k, z.x, ja);
+                        sb[i] += growth_term_r(gr, k, z.y, jb);
+                        if (saveU) {
+                            LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
                             su[0] = z.x;
                             su[W] = z.y;
                         }
@@ -304,8 +316,8 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
                     int ya = 2 * (rp0 + p);
                     float a2, b2;
-                    if (wd.saveA) {
-                        LENIA_GLOBAL float *sa0 = gp(wd.saveA) + (size_t)ya * W + x;
+                    if (saveA) {
+                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
                         sa0[0] = ar0[i];
                         sa0[W] = ar1[i];
                     }
@@ -313,8 +325,8 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                         a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
                         b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
                     } else {
-                        a2 = growth_plain(g, ar0[i], sa[i]);
-                        b2 = growth_plain(g, ar1[i], sb[i]);
+                        a2 = growth_plain_r(gr, ar0[i], sa[i]);
+                        b2 = growth_plain_r(gr, ar1[i], sb[i]);
                     }
                     int ia = 2 * p * W + x;
                     Arow[ia] = a2;
@@ -326,7 +338,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
         __syncthreads();
         forward_rows<T>(Z, wd, tw, rp0, nt, tid);
         __syncthreads();
-    }
+    } while (0);
 }
 
 // ---------------------------------------------------------------- column pass
@@ -336,7 +348,8 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                                                 const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
-    for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
+    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
+    if (ti < nTiles) do {
         const int2 tile = tiles[tile_of(ti, nTiles)];
         const FftWorldDev &wd = worlds[tile.x];
         const int H = wd.H, Wh = wd.Wh;
@@ -380,7 +393,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                 }
             }
             __syncthreads();
-            fft_lds<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
             if (wd.saveAhat) {
                 LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
                 for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
@@ -401,7 +414,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                 if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
             }
             __syncthreads();
-            fft_lds<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
+            fft_run<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
             for (int e = tid; e < nk; e += T) {
                 int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
                 gst2(G0 + (size_t)k * sStride + c, Y[sw(e)]);
@@ -416,7 +429,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             if (e < ne) sr[i] = S0[e];
         }
         __syncthreads();
-        fft_lds<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+        fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
         float2 xs[kMaxV];
 #pragma unroll
         for (int i = 0; i < kMaxV; ++i) {
@@ -451,7 +464,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
             __syncthreads();
         }
-    }
+    } while (0);
 }
 
 // ---------------------------------------------------------------- set-up kernels
@@ -548,6 +561,7 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                            size_t lds, uint32_t stepOff, int kSel, float *out,
                            hipStream_t s) {
+    if (grid < nTiles) return hipErrorInvalidValue;  // one tile per workgroup
     switch (threads) {
         case 64: rows_dispatch<64>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 128: rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
@@ -569,6 +583,7 @@ static void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tile
 
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
                            int nTiles, int grid, size_t lds, hipStream_t s) {
+    if (grid < nTiles) return hipErrorInvalidValue;  // one tile per workgroup
     switch (threads) {
         case 64: cols_dispatch<64>(kpar, worlds, tiles, nTiles, grid, lds, s); break;
         case 128: cols_dispatch<128>(kpar, worlds, tiles, nTiles, grid, lds, s); break;

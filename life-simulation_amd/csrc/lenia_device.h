@@ -161,6 +161,48 @@ __device__ __forceinline__ float growth_plain(const GrowthParams &g, float A, fl
     return fminf(fmaxf(A2, 0.0f), 1.0f);
 }
 
+// The growth constants of a world copied into registers once per tile, so the
+// per-cell loops neither reload them after each barrier nor loop over a
+// runtime kernel count.  Same arithmetic as growth_term / growth_plain.
+struct GrowthRegs {
+    float mu[4], scale[4], weight[4];
+    float dt, sumWeights;
+    uint32_t mode;
+    int n;
+};
+
+__device__ __forceinline__ GrowthRegs growth_regs(const GrowthParams &g) {
+    GrowthRegs q;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        q.mu[k] = g.mu[k];
+        q.scale[k] = g.bellScale[k];
+        q.weight[k] = g.weight[k];
+    }
+    q.dt = g.dt;
+    q.sumWeights = g.sumWeights;
+    q.mode = g.growthMode;
+    q.n = (int)g.nKernels;
+    return q;
+}
+
+__device__ __forceinline__ float growth_term_r(const GrowthRegs &q, int k, float u, float muJ) {
+    float b = bell_of(u, q.mu[k] + muJ, q.scale[k]);
+    if (q.mode == 1u) return q.weight[k] * b;
+    return q.weight[k] * (2.0f * b - 1.0f);
+}
+
+__device__ __forceinline__ float growth_plain_r(const GrowthRegs &q, float A, float acc) {
+    float A2;
+    if (q.mode == 1u) {
+        float target = (q.sumWeights > 1e-6f) ? (acc / q.sumWeights) : A;
+        A2 = A + q.dt * (target - A);
+    } else {
+        A2 = A + q.dt * acc;
+    }
+    return fminf(fmaxf(A2, 0.0f), 1.0f);
+}
+
 // Step seed of the step a kernel is running: pcg(seed_ ^ (frame*197 + sub))
 // (Modules/FieldModules/Lenia.cpp:564)
 __device__ __forceinline__ uint32_t clock_seed(const StepClock &c, uint32_t moduleSeed) {

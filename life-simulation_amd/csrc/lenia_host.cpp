@@ -241,11 +241,22 @@ bool lenia::plan_make(int n, const int *radix, int ns, FftPlan &p) {
     if (prod != n) return false;
     p.n = n;
     p.nStages = ns;
+    // the radix sequences with a compile-time kernel
+    struct Fixed {
+        int n, ns, r[3];
+        uint32_t code;
+    };
+    static const Fixed fixed[] = {{64, 2, {8, 8, 0}, kPlan64},        {128, 2, {16, 8, 0}, kPlan128},
+                                  {256, 2, {16, 16, 0}, kPlan256},    {512, 3, {8, 8, 8}, kPlan512},
+                                  {1024, 3, {16, 8, 8}, kPlan1024}, {2048, 3, {16, 16, 8}, kPlan2048}};
+    p.code = kPlanGeneric;
+    for (const Fixed &f : fixed)
+        if (f.n == n && f.ns == ns && std::equal(radix, radix + ns, f.r)) p.code = f.code;
     return true;
 }
 
 // radix sequence for n: the power-of-two part in the fewest stages of radix
-// <= maxR (balanced, e.g. 1024 = 16*16*4, 512 = 8*8*8), then 3s and 5s
+// <= maxR (balanced, e.g. 1024 = 16*8*8, 512 = 8*8*8), then 3s and 5s
 bool lenia::plan_default(int n, int maxR, FftPlan &p) {
     int a, b, c;
     if (n < 2 || !factor235(n, a, b, c)) return false;

@@ -51,9 +51,24 @@ struct FftStage {
 };
 
 // FFT plan of one length: stages in order
+// Radix sequences that also have a compile-time kernel (power-of-two lengths
+// with their default plans).  The fixed kernels run the same butterflies in
+// the same order as the table-driven one, so both give the same bits.
+enum : uint32_t {
+    kPlanGeneric = 0,
+    kPlan64 = 1,    // 8 8
+    kPlan128 = 2,   // 16 8
+    kPlan256 = 3,   // 16 16
+    kPlan512 = 4,   // 8 8 8
+    kPlan1024 = 5,  // 16 8 8
+    kPlan2048 = 6,  // 16 16 8
+};
+
 struct FftPlan {
     int32_t n;
     int32_t nStages;
+    uint32_t code;  // kPlan*: which compile-time kernel runs this plan
+    int32_t pad;
     FftStage st[kMaxStages];
 };
 
