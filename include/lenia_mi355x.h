@@ -49,6 +49,8 @@ extern "C" {
 #define LENIA_INIT_PATCH 0 /* seeded uniform[0,1) noise in a rectangle, 0 elsewhere */
 #define LENIA_INIT_NOISE 1 /* leniaInit value-noise soup (Lenia.metal:43-61) */
 #define LENIA_INIT_NONE 2  /* keep the current state (set with lenia_world_write_state) */
+#define LENIA_INIT_STAMP 3 /* leniaStampInit organism stamps (Lenia.metal:242-303); needs
+                              lenia_world_set_organism */
 
 /* one entry of the scene "kernels" array (Lenia.h:99-105) with the radius
  * already resolved (radius * radiusScale) */
@@ -150,6 +152,12 @@ int lenia_world_potential(lenia_world *w, uint32_t k, float *host, size_t count)
  * (LifeCore/Metal/GPUTimer.mm:28-94, Apps/LifeBench/main.cpp:112-133) */
 int lenia_world_pass_times(lenia_world *w, uint32_t nSteps, double *outMs, uint32_t maxPasses,
                            uint32_t *nPasses);
+/* organism cells for LENIA_INIT_STAMP (row-major orgW x orgH, values in
+ * [0,1]; the organism texture of Lenia.cpp:358-371) with the stamp count
+ * (capped at 32) and the rotate flag (stampCount / stampRotate,
+ * Lenia.cpp:283-284); copied to the device, used by every later init */
+int lenia_world_set_organism(lenia_world *w, const float *cells, uint32_t orgW, uint32_t orgH,
+                             uint32_t stampCount, uint32_t stampRotate);
 /* resolved descriptor (tile shapes and FFT plans filled in) */
 int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
 /* empirical launch-shape search for the FFT form (no counterpart in the
@@ -198,6 +206,92 @@ int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
  * pass for all worlds) picked by timing every allowed choice over nSteps steps;
  * states and clocks restored, results unchanged bit for bit */
 int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep);
+/* nFrames consecutive frames frame0, frame0+1, ... of every world, substeps
+ * steps each, with the frame / substep clock of LeniaModule::encode
+ * (Lenia.cpp:562-564) -- what one SceneRunner::step per frame does to each
+ * variant (SceneRunner.cpp:146-246).  Pending inits run in the first frame. */
+int lenia_batch_frames(lenia_batch *b, uint32_t frame0, uint32_t nFrames, uint32_t substeps,
+                       uint32_t graphSteps);
+
+/* ---- colour map: field -> RGBA16F layer (colorMapField / colorMapFieldScaled,
+ * Shaders/Render/ColorMap.metal:17-75; ColorMapParams, LifeCore/Render/ColorMapPass.h:16-25).
+ * rgb = max(a + b cos(2 pi (c t + d)), 0) * exposure, alpha = t, with
+ * t = clamp(v * inputScale + inputBias, 0, 1). */
+typedef struct lenia_colormap {
+    float a[3], b[3], c[3], d[3];
+    float inputScale, inputBias, exposure;
+    uint32_t channel; /* 0: the field value; 1, 2: zero; 3: one (an R32F texel is (r, 0, 0, 1)) */
+} lenia_colormap;
+
+/* the palette a Lenia module starts from (ColorMapPass.h:16-25 defaults with
+ * Lenia.cpp:350-354's d = (0.60, 0.45, 0.30), channel 0, inputScale 1) */
+void lenia_colormap_lenia_defaults(lenia_colormap *p);
+/* field (device, srcW x srcH floats) -> rgba (device, dstW x dstH pixels of 4
+ * half floats), ordered on ctx's stream.  Same size: per-texel read; other
+ * sizes: bilinear at pixel centres with wrap-around addressing. */
+int lenia_colormap_field(lenia_ctx *ctx, const float *field, uint32_t srcW, uint32_t srcH,
+                         const lenia_colormap *p, uint16_t *rgba, uint32_t dstW, uint32_t dstH);
+
+/* ---- scene runner for Lenia scenes: the scene JSON (LifeCore/Params/Scene.cpp:24-113)
+ * through LeniaModule::setup / encode (Lenia.cpp:208-653) and the per-frame
+ * visibility, reseed and freeze rules of SceneRunner::step
+ * (LifeCore/Sim/SceneRunner.cpp:126-215).  Every enabled module must be of
+ * type "Lenia".  Compositing, post effects, audio analysis and output sinks
+ * are not part of this library: a module's output is its colour layer, and
+ * the values the reference's ParameterBus would hand a module each frame are
+ * set with lenia_scene_set_param. */
+typedef struct lenia_scene lenia_scene;
+typedef struct lenia_scene_info {
+    uint32_t width, height; /* "resolution": size of every module's colour layer */
+    uint32_t seed;          /* scene seed */
+    uint32_t substeps;
+    uint32_t frameIndex;    /* frames stepped since the last reset */
+    uint32_t nModules;
+} lenia_scene_info;
+
+/* what LeniaModule::setup resolves for enabled module `module` of a scene
+ * (descriptor, palette, instance name), without touching a GPU.  Set-up
+ * warnings of the reference (fallbacks, truncations) go to stderr as there. */
+int lenia_scene_describe(const char *json, size_t len, uint32_t module, lenia_world_desc *desc,
+                         lenia_colormap *cmap, char *name, size_t nameCap);
+int lenia_scene_create(lenia_ctx *ctx, const char *json, size_t len, uint32_t substeps,
+                       lenia_scene **out);
+int lenia_scene_destroy(lenia_scene *s);
+/* SceneRunner::reset: frame 0, every module reseeded from the scene seed */
+int lenia_scene_reset(lenia_scene *s);
+/* nFrames calls of SceneRunner::step (each: substeps steps of every visible,
+ * unfrozen module, then its colour layer) */
+int lenia_scene_step(lenia_scene *s, uint32_t nFrames);
+/* the per-frame value of "module.key" (ParameterBus::value, base + mappings);
+ * read from the next frame on */
+int lenia_scene_set_param(lenia_scene *s, const char *target, float value);
+int lenia_scene_get_info(lenia_scene *s, lenia_scene_info *out);
+/* module's world (owned by the scene) */
+int lenia_scene_world(lenia_scene *s, uint32_t module, lenia_world **world);
+/* module's colour layer, width*height*4 half floats (host or device copy) */
+int lenia_scene_layer(lenia_scene *s, uint32_t module, uint16_t *rgba, size_t count, int onDevice);
+
+/* ---- parameter sweep on the batched launch (the variant loop of
+ * Apps/LifePresetLab/main.cpp:156-324, 718-806): sweep JSON ("mode"
+ * cartesian | per-axis, "seeds", "axes" of {target, values | range + steps})
+ * expanded into variants of a one-module Lenia scene, all stepped as one
+ * batch; activity (mean / stddev of Rec.709 luma of the colour layer, alive
+ * when 0.01 <= mean <= 0.7 and stddev >= 0.02, main.cpp:67-90) on the GPU. */
+typedef struct lenia_sweep lenia_sweep;
+int lenia_sweep_create(lenia_ctx *ctx, const char *sceneJson, size_t sceneLen, const char *sweepJson,
+                       size_t sweepLen, uint32_t substeps, lenia_sweep **out);
+int lenia_sweep_destroy(lenia_sweep *s);
+int lenia_sweep_count(lenia_sweep *s, uint32_t *nVariants);
+/* frames 0 .. nFrames-1 of every variant (graphSteps as lenia_batch_step) */
+int lenia_sweep_run(lenia_sweep *s, uint32_t nFrames, uint32_t graphSteps);
+/* per-variant activity of the last frame run (arrays of n = count entries) */
+int lenia_sweep_activity(lenia_sweep *s, double *meanLuma, double *stddevLuma, int *alive,
+                         uint32_t n);
+/* variant v's world (owned by the sweep) */
+int lenia_sweep_world(lenia_sweep *s, uint32_t v, lenia_world **world);
+/* variants.json text (id, seed, overrides, activity per variant); *needed is
+ * the size including the terminating 0, buf may be NULL to query it */
+int lenia_sweep_report(lenia_sweep *s, char *buf, size_t cap, size_t *needed);
 
 #ifdef __cplusplus
 }

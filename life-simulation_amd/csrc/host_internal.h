@@ -54,6 +54,9 @@ struct lenia_world {
     float *A0 = nullptr, *A1 = nullptr;
     float *scratchOut = nullptr;
     lenia::StepClock *clock = nullptr;
+    float *org = nullptr;  // organism cells for stamp init
+    int orgW = 0, orgH = 0;
+    uint32_t stampCount = 6, stampRotate = 1;
     void *devDesc = nullptr;
     lenia::FftWorldDev fdev{};
     lenia::DirectWorldDev ddev{};

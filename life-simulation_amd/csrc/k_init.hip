@@ -49,6 +49,45 @@ __global__ void k_init_noise(float *A, int W, int H, uint32_t seed, float cov, f
     A[(size_t)y * W + x] = v * 0.9f;
 }
 
+// organism stamps (leniaStampInit, Shaders/Field/Lenia.metal:242-303): up to
+// 32 copies of the organism's cells at hashed centres, each in one of the 8
+// dihedral orientations when rotation is on; a cell takes the largest stamp
+// value covering it (nearest sampling, toroidal offsets)
+__device__ __forceinline__ float2 stamp_inverse(float2 d, uint32_t variant) {
+    if (variant & 4u) d.x = -d.x;
+    const uint32_t rot = variant & 3u;
+    if (rot == 1u) d = make_float2(d.y, -d.x);
+    else if (rot == 2u) d = make_float2(-d.x, -d.y);
+    else if (rot == 3u) d = make_float2(-d.y, d.x);
+    return d;
+}
+
+__global__ void k_init_stamp(float *A, int W, int H, const float *__restrict__ org, int ow, int oh,
+                             uint32_t count, uint32_t rotate, uint32_t seed) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    float v = 0.0f;
+    const float cx0 = (float)ow * 0.5f, cy0 = (float)oh * 0.5f;
+    const uint32_t n = count < 32u ? count : 32u;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float cx = rand01(i, 0u, 907u, seed) * (float)W;
+        const float cy = rand01(0u, i, 911u, seed) * (float)H;
+        uint32_t variant = 0u;
+        if (rotate) {
+            const uint32_t r = (uint32_t)(rand01(i, 1u, 919u, seed) * 8.0f);
+            variant = r < 7u ? r : 7u;
+        }
+        float dx = (float)x + 0.5f - cx, dy = (float)y + 0.5f - cy;
+        dx -= (float)W * metal_round(dx / (float)W);
+        dy -= (float)H * metal_round(dy / (float)H);
+        const float2 l = stamp_inverse(make_float2(dx, dy), variant);
+        const int lx = (int)floorf(l.x + cx0), ly = (int)floorf(l.y + cy0);
+        if (lx >= 0 && lx < ow && ly >= 0 && ly < oh) v = fmaxf(v, org[(size_t)ly * ow + lx]);
+    }
+    A[(size_t)y * W + x] = fminf(fmaxf(v, 0.0f), 1.0f);
+}
+
 __global__ void k_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t substeps,
                             uint32_t parity) {
     StepClock v;
@@ -85,9 +124,41 @@ hipError_t launch_init_noise(float *A, int W, int H, uint32_t seed, float covera
     return hipGetLastError();
 }
 
+hipError_t launch_init_stamp(float *A, int W, int H, const float *org, int ow, int oh,
+                             uint32_t count, uint32_t rotate, uint32_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_stamp, dim3((W + 255) / 256, H), dim3(256), 0, s, A, W, H, org, ow, oh,
+                       count, rotate, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t substeps,
                             uint32_t parity, hipStream_t s) {
     hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(1), 0, s, c, frame, sub, substeps, parity);
+    return hipGetLastError();
+}
+
+// every world's clock at the start of frame `frame` of `substeps` substeps;
+// the ping-pong parity stays
+template <class WorldT>
+__global__ void k_frame_clocks(const WorldT *worlds, int n, uint32_t frame, uint32_t substeps) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        StepClock *c = worlds[i].clock;
+        c->frame = frame;
+        c->sub = 0u;
+        c->substeps = substeps;
+        c->steps = 0u;
+    }
+}
+
+hipError_t launch_frame_clocks(const FftWorldDev *worlds, int n, uint32_t frame, uint32_t substeps,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_clocks<FftWorldDev>, dim3(1), dim3(256), 0, s, worlds, n, frame, substeps);
+    return hipGetLastError();
+}
+hipError_t launch_frame_clocks(const DirectWorldDev *worlds, int n, uint32_t frame, uint32_t substeps,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_frame_clocks<DirectWorldDev>, dim3(1), dim3(256), 0, s, worlds, n, frame,
+                       substeps);
     return hipGetLastError();
 }
 

@@ -78,6 +78,8 @@ hipError_t launch_init_patch(float *A, int W, int H, uint32_t seed, int x0, int 
                              int ph, hipStream_t s);
 hipError_t launch_init_noise(float *A, int W, int H, uint32_t seed, float coverage, float scale,
                              hipStream_t s);
+hipError_t launch_init_stamp(float *A, int W, int H, const float *org, int ow, int oh,
+                             uint32_t count, uint32_t rotate, uint32_t seed, hipStream_t s);
 hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t substeps,
                             uint32_t parity, hipStream_t s);
 // move the stored clocks of n worlds `by` steps on (after the step kernels
@@ -85,6 +87,21 @@ hipError_t launch_set_clock(StepClock *c, uint32_t frame, uint32_t sub, uint32_t
 hipError_t launch_advance_clocks(const FftWorldDev *worlds, int n, uint32_t by, hipStream_t s);
 hipError_t launch_advance_clocks(const DirectWorldDev *worlds, int n, uint32_t by, hipStream_t s);
 hipError_t launch_advance_clocks(const MfmaWorldDev *worlds, int n, uint32_t by, hipStream_t s);
+// set n worlds' clocks to the start of `frame` with `substeps` substeps
+hipError_t launch_frame_clocks(const FftWorldDev *worlds, int n, uint32_t frame, uint32_t substeps,
+                               hipStream_t s);
+hipError_t launch_frame_clocks(const DirectWorldDev *worlds, int n, uint32_t frame, uint32_t substeps,
+                               hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+
+// colour map and activity (k_colormap.hip).  field srcW x srcH -> RGBA16F
+// dstW x dstH (4 x uint16 per pixel, 8-byte aligned); bilinear when the sizes
+// differ.  luma_stats: Rec.709 luma of n RGBA16F pixels -> {sum, sum of
+// squares} in FP64 at out[0..1] (partial: 2 x nBlocks doubles of scratch).
+hipError_t launch_colormap(const float *field, int srcW, int srcH, const ColorMapParams &p,
+                           uint16_t *rgba, int dstW, int dstH, hipStream_t s);
+int luma_stats_blocks(size_t n);
+hipError_t launch_luma_stats(const uint16_t *rgba, size_t n, double *partial, double *out,
+                             hipStream_t s);
 
 }  // namespace lenia

@@ -388,15 +388,10 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     return LENIA_OK;
 }
 
-extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
-    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: NULL batch");
-    if (graphSteps && nSteps % graphSteps)
-        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: nSteps %u is not a multiple of %u",
-                    nSteps, graphSteps);
+// nSteps steps of every world after batch_prepare (graph replays when
+// graphSteps > 0), bracketed by the context's timing events
+static int batch_run_steps(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
     lenia_ctx *ctx = b->ctx;
-    (void)hipSetDevice(ctx->device);
-    int rc = batch_prepare(b);
-    if (rc) return rc;
     (void)hipEventRecord(ctx->ev0, ctx->stream);
     if (graphSteps) {
         if (!b->graph || b->graphSteps != graphSteps) {
@@ -426,9 +421,45 @@ extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphS
     }
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->timed = true;
-    for (lenia_world *w : b->worlds) {
-        w->hostSteps += nSteps;
-        w->frameNext += nSteps;
-    }
+    for (lenia_world *w : b->worlds) w->hostSteps += nSteps;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: NULL batch");
+    if (graphSteps && nSteps % graphSteps)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_step: nSteps %u is not a multiple of %u",
+                    nSteps, graphSteps);
+    (void)hipSetDevice(b->ctx->device);
+    int rc = batch_prepare(b);
+    if (rc) return rc;
+    rc = batch_run_steps(b, nSteps, graphSteps);
+    if (rc) return rc;
+    for (lenia_world *w : b->worlds) w->frameNext += nSteps;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_frames(lenia_batch *b, uint32_t frame0, uint32_t nFrames, uint32_t substeps,
+                                  uint32_t graphSteps) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_frames: NULL batch");
+    const uint32_t S = std::max(1u, substeps);
+    const uint64_t total = (uint64_t)nFrames * S;
+    if (total > 0xFFFFFFFFull) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_frames: too many steps");
+    const uint32_t nSteps = (uint32_t)total;
+    if (graphSteps && nSteps % graphSteps)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_frames: %u steps are not a multiple of %u",
+                    nSteps, graphSteps);
+    (void)hipSetDevice(b->ctx->device);
+    int rc = batch_prepare(b);
+    if (rc) return rc;
+    // every world's clock at (frame0, sub 0, S substeps); the step offsets
+    // then walk the subs of consecutive frames
+    const int n = (int)b->worlds.size();
+    hipError_t e = b->isFft ? launch_frame_clocks((const FftWorldDev *)b->devWorlds, n, frame0, S, b->ctx->stream)
+                            : launch_frame_clocks((const DirectWorldDev *)b->devWorlds, n, frame0, S, b->ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "frame clocks");
+    rc = batch_run_steps(b, nSteps, graphSteps);
+    if (rc) return rc;
+    for (lenia_world *w : b->worlds) w->frameNext = frame0 + nFrames;
     return LENIA_OK;
 }

@@ -318,7 +318,7 @@ static int check_desc(const lenia_world_desc *d) {
         return fail(LENIA_ERR_INVALID_ARG, "unknown convMode %u", d->convMode);
     if (d->growthMode > 1) return fail(LENIA_ERR_INVALID_ARG, "growthMode must be 0 or 1");
     if (!(d->dt >= 0.0f) || !std::isfinite(d->dt)) return fail(LENIA_ERR_INVALID_ARG, "dt must be finite and >= 0");
-    if (d->initMode > LENIA_INIT_NONE) return fail(LENIA_ERR_INVALID_ARG, "unknown initMode");
+    if (d->initMode > LENIA_INIT_STAMP) return fail(LENIA_ERR_INVALID_ARG, "unknown initMode");
     for (uint32_t k = 0; k < d->nKernels; ++k) {
         const lenia_kernel_desc &kd = d->kernels[k];
         if (kd.shape > LENIA_SHAPE_RING || kd.core > LENIA_CORE_EXP)
@@ -694,7 +694,7 @@ static void world_release(lenia_world *w) {
     world_drop_graph(w);
     world_free_adjoint(w);
     void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF, w->toepHi, w->toepMid, w->toepLo,
-                    w->clock, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut};
+                    w->clock, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut, w->org};
     for (void *p : ptrs) dev_free(ctx, p);
 }
 
@@ -826,6 +826,35 @@ extern "C" int lenia_world_destroy(lenia_world *w) {
     return LENIA_OK;
 }
 
+extern "C" int lenia_world_set_organism(lenia_world *w, const float *cells, uint32_t orgW,
+                                        uint32_t orgH, uint32_t stampCount, uint32_t stampRotate) {
+    if (!w || !cells) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_set_organism: NULL argument");
+    if (orgW < 1 || orgH < 1 || orgW > 4096 || orgH > 4096)
+        return fail(LENIA_ERR_INVALID_ARG, "organism size %ux%u out of range", orgW, orgH);
+    const size_t n = (size_t)orgW * orgH;
+    for (size_t i = 0; i < n; ++i)
+        if (!(cells[i] >= 0.0f && cells[i] <= 1.0f))
+            return fail(LENIA_ERR_INVALID_ARG, "organism cell %zu out of [0,1]", i);
+    lenia_ctx *ctx = w->ctx;
+    (void)hipSetDevice(ctx->device);
+    float *org = nullptr;
+    int rc = dev_alloc(ctx, (void **)&org, n * sizeof(float));
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(org, cells, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        dev_free(ctx, org);
+        return hip_fail(e, "organism upload");
+    }
+    dev_free(ctx, w->org);
+    w->org = org;
+    w->orgW = (int)orgW;
+    w->orgH = (int)orgH;
+    w->stampCount = std::min(stampCount, 32u);
+    w->stampRotate = stampRotate ? 1u : 0u;
+    return LENIA_OK;
+}
+
 extern "C" int lenia_world_reset(lenia_world *w, uint32_t sceneSeed) {
     if (!w) return fail(LENIA_ERR_INVALID_ARG, "lenia_world_reset: NULL world");
     (void)hipSetDevice(w->ctx->device);
@@ -852,6 +881,10 @@ int lenia::world_apply_init(lenia_world *w) {
         e = launch_init_patch(cur, W, H, w->moduleSeed, x0, y0, pw, ph, ctx->stream);
     } else if (d.initMode == LENIA_INIT_NOISE) {
         e = launch_init_noise(cur, W, H, w->moduleSeed, d.initCoverage, d.initScale, ctx->stream);
+    } else if (d.initMode == LENIA_INIT_STAMP) {
+        if (!w->org) return fail(LENIA_ERR_STATE, "stamp init: no organism set (lenia_world_set_organism)");
+        e = launch_init_stamp(cur, W, H, w->org, w->orgW, w->orgH, w->stampCount, w->stampRotate,
+                              w->moduleSeed, ctx->stream);
     }
     if (e != hipSuccess) return hip_fail(e, "init pass");
     w->needsInit = false;

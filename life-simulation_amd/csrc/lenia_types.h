@@ -10,6 +10,14 @@ namespace lenia {
 constexpr int kMaxKernels = 4;
 constexpr int kMaxStages = 8;  // FFT plan: up to 8 radix stages
 
+// cosine-palette colour map (ColorMapParams, LifeCore/Render/ColorMapPass.h:16-25;
+// same layout as lenia_colormap in the C ABI)
+struct ColorMapParams {
+    float a[3], b[3], c[3], d[3];
+    float inputScale, inputBias, exposure;
+    uint32_t channel;
+};
+
 // growth parameters of one world (the LeniaMultiParams fields that the
 // growth rule reads, Shaders/Field/Lenia.metal:138-154)
 struct GrowthParams {

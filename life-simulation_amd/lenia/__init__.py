@@ -12,12 +12,13 @@ import numpy as np
 
 from . import _abi
 from ._abi import (CONV_DIRECT, CONV_DIRECT_BF16, CONV_DIRECT_BF16X3, CONV_FFT, CORE_EXP,
-                   CORE_GAUSSIAN, INIT_NOISE, INIT_NONE, INIT_PATCH, SHAPE_RING, SHAPE_SHELL,
-                   LeniaError, check)
+                   CORE_GAUSSIAN, INIT_NOISE, INIT_NONE, INIT_PATCH, INIT_STAMP, SHAPE_RING,
+                   SHAPE_SHELL, LeniaError, check)
 
-__all__ = ["Context", "World", "Batch", "LeniaError", "CONV_DIRECT", "CONV_FFT",
-           "CONV_DIRECT_BF16", "CONV_DIRECT_BF16X3", "SHAPE_SHELL", "SHAPE_RING",
-           "CORE_GAUSSIAN", "CORE_EXP", "INIT_PATCH", "INIT_NOISE", "INIT_NONE"]
+__all__ = ["Context", "World", "Batch", "Scene", "Sweep", "LeniaError", "describe_scene",
+           "colormap_defaults", "colormap_field", "CONV_DIRECT", "CONV_FFT", "CONV_DIRECT_BF16",
+           "CONV_DIRECT_BF16X3", "SHAPE_SHELL", "SHAPE_RING", "CORE_GAUSSIAN", "CORE_EXP",
+           "INIT_PATCH", "INIT_NOISE", "INIT_NONE", "INIT_STAMP"]
 
 
 def _fptr(a: np.ndarray):
@@ -73,9 +74,29 @@ class World:
         d = _abi.world_desc(desc)
         check(_abi.lib().lenia_world_create(ctx.handle, C.byref(d), C.byref(self._h)))
 
+    @classmethod
+    def _borrow(cls, ctx: Context, handle: int) -> "World":
+        """a non-owning view of a world owned by a scene or sweep"""
+        w = cls.__new__(cls)
+        w.ctx = ctx
+        w._h = C.c_void_p(handle)
+        w._rollouts = 0
+        w._owned = False
+        d = _abi.WorldDesc()
+        check(_abi.lib().lenia_world_get_desc(w._h, C.byref(d)))
+        w.desc = _abi.desc_dict(d)
+        w.W, w.H = int(d.width), int(d.height)
+        return w
+
     @property
     def handle(self):
         return self._h
+
+    def set_organism(self, cells, stamp_count: int = 6, stamp_rotate: bool = True):
+        """organism cells (rows x cols in [0,1]) for INIT_STAMP"""
+        a = np.ascontiguousarray(cells, dtype=np.float32)
+        check(_abi.lib().lenia_world_set_organism(self._h, _fptr(a), a.shape[1], a.shape[0],
+                                                  stamp_count, 1 if stamp_rotate else 0))
 
     def reset(self, scene_seed: int):
         check(_abi.lib().lenia_world_reset(self._h, scene_seed & 0xFFFFFFFF))
@@ -162,9 +183,9 @@ class World:
                 "rowThreads": d.rowThreads, "colThreads": d.colThreads}
 
     def close(self):
-        if self._h:
+        if self._h and getattr(self, "_owned", True):
             _abi.lib().lenia_world_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __enter__(self):
         return self
@@ -194,6 +215,10 @@ class Batch:
     def step(self, n: int, graph_steps: int = 0):
         check(_abi.lib().lenia_batch_step(self._h, n, graph_steps))
 
+    def frames(self, frame0: int, n_frames: int, substeps: int = 1, graph_steps: int = 0):
+        """n_frames consecutive frames of `substeps` steps each, from frame0"""
+        check(_abi.lib().lenia_batch_frames(self._h, frame0, n_frames, substeps, graph_steps))
+
     def autotune(self, n_steps: int = 10) -> float:
         """time every launch shape valid for all worlds, keep the fastest;
         returns the tuned kernel ms per step (states and clocks restored)"""
@@ -204,6 +229,129 @@ class Batch:
     def close(self):
         if self._h:
             _abi.lib().lenia_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def describe_scene(text: str, module: int = 0):
+    """(descriptor dict, colour-map dict, module name) that module `module`
+    of a Lenia scene JSON resolves to; no GPU needed"""
+    b = text.encode()
+    d, c = _abi.WorldDesc(), _abi.ColorMap()
+    name = C.create_string_buffer(256)
+    check(_abi.lib().lenia_scene_describe(b, len(b), module, C.byref(d), C.byref(c), name, 256))
+    return _abi.desc_dict(d), _abi.colormap_dict(c), name.value.decode()
+
+
+def colormap_defaults() -> dict:
+    c = _abi.ColorMap()
+    _abi.lib().lenia_colormap_lenia_defaults(C.byref(c))
+    return _abi.colormap_dict(c)
+
+
+def colormap_field(ctx: Context, field_ptr: int, src_w: int, src_h: int, params: dict,
+                   rgba_ptr: int, dst_w: int, dst_h: int) -> None:
+    """device field (src_h x src_w float32) -> device RGBA16F (dst_h x dst_w x 4)"""
+    c = _abi.colormap_struct(params)
+    check(_abi.lib().lenia_colormap_field(ctx.handle, C.c_void_p(field_ptr), src_w, src_h,
+                                          C.byref(c), C.c_void_p(rgba_ptr), dst_w, dst_h))
+
+
+class Scene:
+    """A Lenia scene JSON stepped frame by frame (SceneRunner for Lenia modules)."""
+
+    def __init__(self, ctx: Context, text: str, substeps: int = 1):
+        self.ctx = ctx
+        b = text.encode()
+        self._h = C.c_void_p()
+        check(_abi.lib().lenia_scene_create(ctx.handle, b, len(b), substeps, C.byref(self._h)))
+
+    def info(self) -> dict:
+        i = _abi.SceneInfo()
+        check(_abi.lib().lenia_scene_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _abi.SceneInfo._fields_}
+
+    def reset(self):
+        check(_abi.lib().lenia_scene_reset(self._h))
+
+    def step(self, n_frames: int = 1):
+        check(_abi.lib().lenia_scene_step(self._h, n_frames))
+
+    def set_param(self, target: str, value: float):
+        check(_abi.lib().lenia_scene_set_param(self._h, target.encode(), float(value)))
+
+    def world(self, module: int = 0) -> World:
+        h = C.c_void_p()
+        check(_abi.lib().lenia_scene_world(self._h, module, C.byref(h)))
+        return World._borrow(self.ctx, h.value)
+
+    def layer(self, module: int = 0) -> np.ndarray:
+        """the module's colour layer, (height, width, 4) float16"""
+        i = self.info()
+        out = np.empty((i["height"], i["width"], 4), np.float16)
+        check(_abi.lib().lenia_scene_layer(self._h, module, out.ctypes.data_as(C.c_void_p),
+                                           out.size, 0))
+        return out
+
+    def close(self):
+        if self._h:
+            _abi.lib().lenia_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Sweep:
+    """Variants of a one-module Lenia scene (PresetLab sweep JSON) on the batched launch."""
+
+    def __init__(self, ctx: Context, scene_text: str, sweep_text: str, substeps: int = 1):
+        self.ctx = ctx
+        a, b = scene_text.encode(), sweep_text.encode()
+        self._h = C.c_void_p()
+        check(_abi.lib().lenia_sweep_create(ctx.handle, a, len(a), b, len(b), substeps,
+                                            C.byref(self._h)))
+        n = C.c_uint32()
+        check(_abi.lib().lenia_sweep_count(self._h, C.byref(n)))
+        self.n = n.value
+
+    def run(self, n_frames: int, graph_steps: int = 0):
+        check(_abi.lib().lenia_sweep_run(self._h, n_frames, graph_steps))
+
+    def activity(self):
+        """(mean luma, stddev luma, alive) arrays of the last run"""
+        m = (C.c_double * self.n)()
+        sd = (C.c_double * self.n)()
+        al = (C.c_int * self.n)()
+        check(_abi.lib().lenia_sweep_activity(self._h, m, sd, al, self.n))
+        return np.array(m[:]), np.array(sd[:]), np.array(al[:], bool)
+
+    def world(self, v: int) -> World:
+        h = C.c_void_p()
+        check(_abi.lib().lenia_sweep_world(self._h, v, C.byref(h)))
+        if not h.value:
+            raise LeniaError(-1, f"variant {v} has no world (see its error in the report)")
+        return World._borrow(self.ctx, h.value)
+
+    def report(self) -> dict:
+        import json
+        need = C.c_size_t()
+        check(_abi.lib().lenia_sweep_report(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        check(_abi.lib().lenia_sweep_report(self._h, buf, need.value, C.byref(need)))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if self._h:
+            _abi.lib().lenia_sweep_destroy(self._h)
             self._h = C.c_void_p()
 
     def __enter__(self):

@@ -23,7 +23,7 @@ ERR_STATE = -5
 CONV_DIRECT, CONV_FFT, CONV_DIRECT_BF16, CONV_DIRECT_BF16X3 = 0, 1, 2, 3
 SHAPE_SHELL, SHAPE_RING = 0, 1
 CORE_GAUSSIAN, CORE_EXP = 0, 1
-INIT_PATCH, INIT_NOISE, INIT_NONE = 0, 1, 2
+INIT_PATCH, INIT_NOISE, INIT_NONE, INIT_STAMP = 0, 1, 2, 3
 
 # every symbol include/lenia_mi355x.h declares
 EXPORTS = [
@@ -36,7 +36,12 @@ EXPORTS = [
     "lenia_world_get_desc", "lenia_world_pass_times",
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
     "lenia_rollout_forward", "lenia_rollout_backward",
-    "lenia_world_autotune", "lenia_batch_autotune",
+    "lenia_world_autotune", "lenia_batch_autotune", "lenia_world_set_organism",
+    "lenia_batch_frames", "lenia_colormap_lenia_defaults", "lenia_colormap_field",
+    "lenia_scene_describe", "lenia_scene_create", "lenia_scene_destroy", "lenia_scene_reset",
+    "lenia_scene_step", "lenia_scene_set_param", "lenia_scene_get_info", "lenia_scene_world",
+    "lenia_scene_layer", "lenia_sweep_create", "lenia_sweep_destroy", "lenia_sweep_count",
+    "lenia_sweep_run", "lenia_sweep_activity", "lenia_sweep_world", "lenia_sweep_report",
 ]
 
 
@@ -82,6 +87,30 @@ class WorldDesc(C.Structure):
         ("colsPerTile", C.c_uint32),
         ("rowThreads", C.c_uint32),
         ("colThreads", C.c_uint32),
+    ]
+
+
+class ColorMap(C.Structure):
+    _fields_ = [
+        ("a", C.c_float * 3),
+        ("b", C.c_float * 3),
+        ("c", C.c_float * 3),
+        ("d", C.c_float * 3),
+        ("inputScale", C.c_float),
+        ("inputBias", C.c_float),
+        ("exposure", C.c_float),
+        ("channel", C.c_uint32),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("substeps", C.c_uint32),
+        ("frameIndex", C.c_uint32),
+        ("nModules", C.c_uint32),
     ]
 
 
@@ -150,6 +179,28 @@ def lib():
         "lenia_world_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
+        "lenia_world_set_organism": (i32, [vp, fp, u32, u32, u32, u32]),
+        "lenia_batch_frames": (i32, [vp, u32, u32, u32, u32]),
+        "lenia_colormap_lenia_defaults": (None, [C.POINTER(ColorMap)]),
+        "lenia_colormap_field": (i32, [vp, vp, u32, u32, C.POINTER(ColorMap), vp, u32, u32]),
+        "lenia_scene_describe": (i32, [C.c_char_p, sz, u32, C.POINTER(WorldDesc), C.POINTER(ColorMap),
+                                       C.c_char_p, sz]),
+        "lenia_scene_create": (i32, [vp, C.c_char_p, sz, u32, C.POINTER(vp)]),
+        "lenia_scene_destroy": (i32, [vp]),
+        "lenia_scene_reset": (i32, [vp]),
+        "lenia_scene_step": (i32, [vp, u32]),
+        "lenia_scene_set_param": (i32, [vp, C.c_char_p, C.c_float]),
+        "lenia_scene_get_info": (i32, [vp, C.POINTER(SceneInfo)]),
+        "lenia_scene_world": (i32, [vp, u32, C.POINTER(vp)]),
+        "lenia_scene_layer": (i32, [vp, u32, vp, sz, i32]),
+        "lenia_sweep_create": (i32, [vp, C.c_char_p, sz, C.c_char_p, sz, u32, C.POINTER(vp)]),
+        "lenia_sweep_destroy": (i32, [vp]),
+        "lenia_sweep_count": (i32, [vp, C.POINTER(C.c_uint32)]),
+        "lenia_sweep_run": (i32, [vp, u32, u32]),
+        "lenia_sweep_activity": (i32, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int), u32]),
+        "lenia_sweep_world": (i32, [vp, u32, C.POINTER(vp)]),
+        "lenia_sweep_report": (i32, [vp, C.c_char_p, sz, C.POINTER(sz)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -179,6 +230,40 @@ def kernel_desc(k: dict) -> KernelDesc:
     d.sigma = float(k.get("sigma", 0.017))
     d.weight = float(k.get("weight", 1.0))
     return d
+
+
+def desc_dict(d: WorldDesc) -> dict:
+    """The dict form of a WorldDesc (inverse of world_desc)."""
+    ks = []
+    for i in range(d.nKernels):
+        k = d.kernels[i]
+        ks.append({"shape": k.shape, "core": k.core, "radius": k.radius, "shellMu": k.shellMu,
+                   "shellSigma": k.shellSigma, "betas": [k.betas[j] for j in range(k.nBetas)],
+                   "mu": k.mu, "sigma": k.sigma, "weight": k.weight})
+    out = {f: getattr(d, f) for f, _ in WorldDesc._fields_ if f != "kernels"}
+    out["kernels"] = ks
+    return out
+
+
+def colormap_dict(c: ColorMap) -> dict:
+    return {"a": list(c.a), "b": list(c.b), "c": list(c.c), "d": list(c.d),
+            "inputScale": c.inputScale, "inputBias": c.inputBias, "exposure": c.exposure,
+            "channel": c.channel}
+
+
+def colormap_struct(p: dict) -> ColorMap:
+    c = ColorMap()
+    lib().lenia_colormap_lenia_defaults(C.byref(c))
+    for key in ("a", "b", "c", "d"):
+        if key in p:
+            for i in range(3):
+                getattr(c, key)[i] = float(p[key][i])
+    for key in ("inputScale", "inputBias", "exposure"):
+        if key in p:
+            setattr(c, key, float(p[key]))
+    if "channel" in p:
+        c.channel = int(p["channel"])
+    return c
 
 
 def world_desc(w: dict) -> WorldDesc:

@@ -213,6 +213,96 @@ void lo_init_value_noise(int W, int H, uint32_t seed, double cov, double scale, 
 }
 
 /* ------------------------------------------------------------------ */
+/* organism stamps: leniaStampInit (Lenia.metal:242-303).  Stamp i sits  */
+/* at (rand01(i,0,907), rand01(0,i,911)) * (W,H); with rotation on, its  */
+/* orientation is one of 8 (rotate 0/90/180/270 x flip-x) from          */
+/* rand01(i,1,919).  A cell takes the largest organism value of the     */
+/* stamps covering it (nearest organism cell, toroidal offsets).  The    */
+/* geometry is FP32 as in the shader, so cell boundaries match it.      */
+/* ------------------------------------------------------------------ */
+static void stamp_inverse(float *dx, float *dy, uint32_t variant) {
+    float x = *dx, y = *dy;
+    if (variant & 4u) x = -x;
+    uint32_t rot = variant & 3u;
+    float nx = x, ny = y;
+    if (rot == 1u) { nx = y; ny = -x; }
+    else if (rot == 2u) { nx = -x; ny = -y; }
+    else if (rot == 3u) { nx = -y; ny = x; }
+    *dx = nx;
+    *dy = ny;
+}
+
+void lo_init_stamp(int W, int H, const double *org, int ow, int oh, uint32_t count, uint32_t rotate,
+                   uint32_t seed, double *out) {
+    uint32_t n = count < 32u ? count : 32u;
+    float cx0 = (float)ow * 0.5f, cy0 = (float)oh * 0.5f;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            double v = 0.0;
+            for (uint32_t i = 0; i < n; ++i) {
+                float cx = (float)lo_rand01(i, 0u, 907u, seed) * (float)W;
+                float cy = (float)lo_rand01(0u, i, 911u, seed) * (float)H;
+                uint32_t variant = 0u;
+                if (rotate) {
+                    uint32_t r = (uint32_t)((float)lo_rand01(i, 1u, 919u, seed) * 8.0f);
+                    variant = r < 7u ? r : 7u;
+                }
+                float dx = (float)x + 0.5f - cx, dy = (float)y + 0.5f - cy;
+                dx -= (float)W * roundf(dx / (float)W);
+                dy -= (float)H * roundf(dy / (float)H);
+                stamp_inverse(&dx, &dy, variant);
+                int lx = (int)floorf(dx + cx0), ly = (int)floorf(dy + cy0);
+                if (lx >= 0 && lx < ow && ly >= 0 && ly < oh) {
+                    double o = org[(size_t)ly * ow + lx];
+                    if (o > v) v = o;
+                }
+            }
+            out[(size_t)y * W + x] = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* colour layer: colorMapField / colorMapFieldScaled                   */
+/* (ColorMap.metal:17-75): t = clamp(v*scale + bias, 0, 1);            */
+/* rgb = max(a + b cos(2 pi (c t + d)), 0) * exposure; alpha = t.  An   */
+/* R32F texel reads (r, 0, 0, 1).  Other sizes: linear filter at the    */
+/* output pixel centre, repeat addressing.                              */
+/* ------------------------------------------------------------------ */
+void lo_colormap(const double *field, int srcW, int srcH, const double *a, const double *b,
+                 const double *c, const double *d, double inputScale, double inputBias,
+                 double exposure, int channel, double *rgba, int dstW, int dstH) {
+    const double twoPi = 6.283185307179586;
+    for (int y = 0; y < dstH; ++y)
+        for (int x = 0; x < dstW; ++x) {
+            double v;
+            if (srcW == dstW && srcH == dstH) {
+                v = field[(size_t)y * srcW + x];
+            } else {
+                double sx = (x + 0.5) / dstW * srcW - 0.5, sy = (y + 0.5) / dstH * srcH - 0.5;
+                double fx0 = floor(sx), fy0 = floor(sy);
+                double ax = sx - fx0, ay = sy - fy0;
+                int x0 = (int)fx0 % srcW, y0 = (int)fy0 % srcH;
+                if (x0 < 0) x0 += srcW;
+                if (y0 < 0) y0 += srcH;
+                int x1 = (x0 + 1) % srcW, y1 = (y0 + 1) % srcH;
+                double v00 = field[(size_t)y0 * srcW + x0], v01 = field[(size_t)y0 * srcW + x1];
+                double v10 = field[(size_t)y1 * srcW + x0], v11 = field[(size_t)y1 * srcW + x1];
+                double top = v00 + ax * (v01 - v00), bot = v10 + ax * (v11 - v10);
+                v = top + ay * (bot - top);
+            }
+            double sel = channel == 0 ? v : (channel >= 3 ? 1.0 : 0.0);
+            double t = sel * inputScale + inputBias;
+            t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+            double *o = rgba + 4 * ((size_t)y * dstW + x);
+            for (int k = 0; k < 3; ++k) {
+                double e = a[k] + b[k] * cos(twoPi * (c[k] * t + d[k]));
+                o[k] = (e > 0.0 ? e : 0.0) * exposure;
+            }
+            o[3] = t;
+        }
+}
+
+/* ------------------------------------------------------------------ */
 /* direct potential: leniaStep's gather (Lenia.metal:73-83)             */
 /* u(x,y) = sum_{dy,dx} K[dy][dx] * A[wrap(y+dy)][wrap(x+dx)]           */
 /* ------------------------------------------------------------------ */

@@ -95,6 +95,18 @@ int lo_rollout(const lo_world *w, const double *A0, int T, int use_fft, double *
 int lo_rollout_grad(const lo_world *w, const double *A0, int T, const double *gT, double *gA0,
                     double *gMu, double *gSigma, double *gWeight, double *gBetas, double *gDt);
 
+/* ---- colour layer and activity ---- */
+/* cosine palette of colorMapField / colorMapFieldScaled (Shaders/Render/ColorMap.metal:17-75):
+ * field srcW x srcH -> rgba dstW x dstH x 4 (doubles); a[3] b[3] c[3] d[3] palette,
+ * inputScale, inputBias, exposure, channel.  Sizes differ: bilinear at pixel centres, wrap. */
+void lo_colormap(const double *field, int srcW, int srcH, const double *a, const double *b,
+                 const double *c, const double *d, double inputScale, double inputBias,
+                 double exposure, int channel, double *rgba, int dstW, int dstH);
+/* organism stamps (leniaStampInit, Shaders/Field/Lenia.metal:242-303); the stamp geometry in
+ * FP32 as the shader computes it */
+void lo_init_stamp(int W, int H, const double *org, int ow, int oh, uint32_t count, uint32_t rotate,
+                   uint32_t seed, double *out);
+
 /* OpenMP thread count for the oracle's parallel loops (<=0: leave default) */
 void lo_threads(int n);
 int lo_max_threads(void);

@@ -108,6 +108,9 @@ def lib():
         L.lo_rollout_grad.restype = C.c_int
         L.lo_rollout_grad.argtypes = [C.POINTER(LoWorld), f64p, C.c_int, f64p, f64p,
                                       f64p, f64p, f64p, f64p, f64p]
+        L.lo_init_stamp.argtypes = [C.c_int, C.c_int, f64p, C.c_int, C.c_int, u32, u32, u32, f64p]
+        L.lo_colormap.argtypes = [f64p, C.c_int, C.c_int, f64p, f64p, f64p, f64p, C.c_double,
+                                  C.c_double, C.c_double, C.c_int, f64p, C.c_int, C.c_int]
         L.lo_threads.argtypes = [C.c_int]
         L.lo_max_threads.restype = C.c_int
         _lib = L
@@ -229,6 +232,37 @@ def init_value_noise(W, H, seed, coverage=0.4, scale=24.0) -> np.ndarray:
     out = np.zeros((H, W))
     lib().lo_init_value_noise(W, H, seed, coverage, scale, _p(out))
     return out
+
+
+def init_stamp(W, H, org, count, rotate, seed) -> np.ndarray:
+    """leniaStampInit (Shaders/Field/Lenia.metal:242-303); org is rows x cols"""
+    o = _c64(org)
+    out = np.zeros((H, W))
+    lib().lo_init_stamp(W, H, _p(o), o.shape[1], o.shape[0], count, 1 if rotate else 0, seed, _p(out))
+    return out
+
+
+def colormap(field, params: dict, dst_w: int | None = None, dst_h: int | None = None) -> np.ndarray:
+    """colorMapField / colorMapFieldScaled (Shaders/Render/ColorMap.metal:17-75) in FP64:
+    (dst_h, dst_w, 4) rgba"""
+    f = _c64(field)
+    H, W = f.shape
+    dw, dh = dst_w or W, dst_h or H
+    out = np.zeros((dh, dw, 4))
+    vec = {k: (C.c_double * 3)(*[float(x) for x in params[k]]) for k in "abcd"}
+    lib().lo_colormap(_p(f), W, H, vec["a"], vec["b"], vec["c"], vec["d"],
+                      float(params.get("inputScale", 1.0)), float(params.get("inputBias", 0.0)),
+                      float(params.get("exposure", 1.0)), int(params.get("channel", 0)), _p(out), dw, dh)
+    return out
+
+
+def luma_stats(rgba) -> tuple[float, float]:
+    """computeLumaStats (Apps/LifePresetLab/main.cpp:67-86) of an RGBA layer"""
+    px = np.asarray(rgba, np.float64).reshape(-1, 4)
+    lum = 0.2126 * px[:, 0] + 0.7152 * px[:, 1] + 0.0722 * px[:, 2]
+    mean = float(lum.sum() / lum.size)
+    var = float((lum * lum).sum() / lum.size - mean * mean)
+    return mean, max(var, 0.0) ** 0.5
 
 
 def potential_direct(A, win) -> np.ndarray:
