@@ -141,6 +141,12 @@ int lenia_world_set_params(lenia_world *w, const lenia_world_desc *desc);
  * ResourcePool.mm:175-191).  Uploading clears the pending init. */
 int lenia_world_read_state(lenia_world *w, float *host, size_t count);
 int lenia_world_write_state(lenia_world *w, const float *host, size_t count);
+/* field snapshot as a .npy file: float32 little endian, C order, shape
+ * (height, width) -- the NumPy format of the reference's state dumps
+ * (LifeCore/IO/NpyWriter.cpp:42-171).  Loading checks dtype, order and shape
+ * and, like lenia_world_write_state, clears the pending init. */
+int lenia_world_save_npy(lenia_world *w, const char *path);
+int lenia_world_load_npy(lenia_world *w, const char *path);
 /* device pointer of the current state buffer (valid until the next step) */
 int lenia_world_state_device(lenia_world *w, float **dev);
 /* potential u_k = K_k (x) A of the current state (the potential_k textures of

@@ -123,6 +123,12 @@ class World:
             raise ValueError(f"state shape {a.shape} != {(self.H, self.W)}")
         check(_abi.lib().lenia_world_write_state(self._h, _fptr(a), a.size))
 
+    def save_npy(self, path: str) -> None:
+        check(_abi.lib().lenia_world_save_npy(self._h, str(path).encode()))
+
+    def load_npy(self, path: str) -> None:
+        check(_abi.lib().lenia_world_load_npy(self._h, str(path).encode()))
+
     def potential(self, k: int = 0) -> np.ndarray:
         out = np.empty((self.H, self.W), np.float32)
         check(_abi.lib().lenia_world_potential(self._h, k, _fptr(out), out.size))

@@ -37,6 +37,7 @@ EXPORTS = [
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
     "lenia_rollout_forward", "lenia_rollout_backward",
     "lenia_world_autotune", "lenia_batch_autotune", "lenia_world_set_organism",
+    "lenia_world_save_npy", "lenia_world_load_npy",
     "lenia_batch_frames", "lenia_colormap_lenia_defaults", "lenia_colormap_field",
     "lenia_scene_describe", "lenia_scene_create", "lenia_scene_destroy", "lenia_scene_reset",
     "lenia_scene_step", "lenia_scene_set_param", "lenia_scene_get_info", "lenia_scene_world",
@@ -180,6 +181,8 @@ def lib():
         "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
         "lenia_world_set_organism": (i32, [vp, fp, u32, u32, u32, u32]),
+        "lenia_world_save_npy": (i32, [vp, C.c_char_p]),
+        "lenia_world_load_npy": (i32, [vp, C.c_char_p]),
         "lenia_batch_frames": (i32, [vp, u32, u32, u32, u32]),
         "lenia_colormap_lenia_defaults": (None, [C.POINTER(ColorMap)]),
         "lenia_colormap_field": (i32, [vp, vp, u32, u32, C.POINTER(ColorMap), vp, u32, u32]),

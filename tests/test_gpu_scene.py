@@ -297,3 +297,50 @@ def test_sweep_errors(ctx):
         rep = sw.report()
         assert "unknown module" in rep["variants"][0]["error"]
         assert rep["variants"][0]["activity"]["alive"] is False
+
+
+def test_npy_state_roundtrip(ctx, tmp_path):
+    """.npy snapshots (NpyWriter.cpp's NumPy format): numpy reads what the
+    library writes and the library loads what numpy writes, bit for bit; bad
+    files are refused with the world unchanged"""
+    import lenia
+    from lenia_cases import config4
+
+    d = config4()
+    with lenia.World(ctx, d) as w:
+        w.reset(31)
+        w.frame(0, 2)
+        A = w.read()
+        p = tmp_path / "state.npy"
+        w.save_npy(p)
+        raw = p.read_bytes()
+        assert raw[:6] == b"\x93NUMPY" and (10 + int.from_bytes(raw[8:10], "little")) % 64 == 0
+        B = np.load(p)
+        assert B.dtype == np.float32 and B.shape == A.shape and np.array_equal(A, B)
+        rng = np.random.default_rng(2)
+        C_ = rng.random(A.shape, dtype=np.float32)
+        q = tmp_path / "in.npy"
+        np.save(q, C_)
+        w.load_npy(q)
+        assert np.array_equal(w.read(), C_)
+        # stepping from a loaded state equals stepping from the same write
+        w.step(3)
+        after = w.read()
+    with lenia.World(ctx, d) as w2:
+        w2.reset(31)
+        w2.write(C_)
+        w2.step(3)
+        assert np.array_equal(w2.read(), after)
+        for bad in (C_.astype(np.float64), C_[:, :-1], np.asfortranarray(C_.T).T.copy(order="F"),
+                    C_.astype(">f4")):
+            r = tmp_path / "bad.npy"
+            np.save(r, bad)
+            before = w2.read()
+            with pytest.raises(lenia.LeniaError):
+                w2.load_npy(r)
+            assert np.array_equal(w2.read(), before)
+        r = tmp_path / "short.npy"
+        np.save(r, C_)
+        r.write_bytes(r.read_bytes()[:-8])
+        with pytest.raises(lenia.LeniaError):
+            w2.load_npy(r)
