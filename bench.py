@@ -252,6 +252,9 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the 256-world batched sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the rollout-gradient leg")
     ap.add_argument("--no-direct", action="store_true", help="skip the 1280x720 direct-forms leg")
+    ap.add_argument("--tiles", default="", metavar="ROWT,ROWP,COLT,COLC",
+                    help="fixed launch shapes of the headline world instead of autotuning "
+                         "(row threads, row pairs per tile, column threads, columns per tile)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming kernel over MIB MiB (counter scaling)")
     args = ap.parse_args()
@@ -268,12 +271,16 @@ def main() -> int:
     from lenia_cases import config1
 
     desc = config1()
+    if args.tiles:
+        rt, rp, ct, cc = (int(x) for x in args.tiles.split(","))
+        desc.update(rowThreads=rt, rowPairsPerTile=rp, colThreads=ct, colsPerTile=cc)
     W, H, K = desc["width"], desc["height"], len(desc["kernels"])
     cells = W * H
     with lenia.Context(0) as ctx, lenia.World(ctx, desc) as w:
         w.reset(7777)
         w.frame(0, 1)
-        w.autotune(20)  # launch shapes for this world (bits unchanged)
+        if not args.tiles:
+            w.autotune(20)  # launch shapes for this world (bits unchanged)
         w.step(args.warmup)
         ctx.sync()
         t0 = time.perf_counter()
@@ -316,7 +323,7 @@ def main() -> int:
                    "fftPlanW": tiles["fftPlanW"], "fftPlanH": tiles["fftPlanH"],
                    "rowPairsPerTile": tiles["rowPairsPerTile"],
                    "colsPerTile": tiles["colsPerTile"], "rowThreads": tiles["rowThreads"],
-                   "colThreads": tiles["colThreads"], "autotuned": True},
+                   "colThreads": tiles["colThreads"], "autotuned": not args.tiles},
         "passes_ms": {k: round(v, 5) for k, v in per.items()},
         "roofline": {"bound": "hbm", "kernel": dom_kernel,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
