@@ -290,15 +290,15 @@ __device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt
     }
 }
 
-// The same Stockham stage with the plan constants known at compile time
-// (power-of-two radices and lengths): index arithmetic folds to shifts and
-// masks, the stage tables need no loads and there is no radix switch.  The
+// The same Stockham stage with the plan constants known at compile time:
+// index arithmetic folds to shifts and masks (multiply-high for the factors 3
+// and 5), the stage tables need no loads and there is no radix switch.  The
 // butterflies, twiddles and stores are those of fft_stage, so the bits are too.
 template <int R, bool INV, int MAXV, int T, int N, int NS, int TWOFF>
 __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *tw, int tid) {
     constexpr int NR = N / R;
     constexpr int MB = MAXV / R;
-    static_assert((NR & (NR - 1)) == 0 && (NS & (NS - 1)) == 0, "power-of-two plans only");
+    static_assert(N % (NS * R) == 0, "radix sequence does not divide the length");
     if constexpr (MB > 0) {
         const int nb = nt * NR;
         float2 v[MB][R];
@@ -354,6 +354,13 @@ __device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt
         case kPlan512: fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, nt, tw, tid); break;
         case kPlan1024: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, nt, tw, tid); break;
         case kPlan2048: fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, nt, tw, tid); break;
+        case kPlan96: fft_fixed<INV, MAXV, T, 96, 1, 0, 8, 4, 3>(buf, nt, tw, tid); break;
+        case kPlan192: fft_fixed<INV, MAXV, T, 192, 1, 0, 8, 8, 3>(buf, nt, tw, tid); break;
+        case kPlan384: fft_fixed<INV, MAXV, T, 384, 1, 0, 16, 8, 3>(buf, nt, tw, tid); break;
+        case kPlan720: fft_fixed<INV, MAXV, T, 720, 1, 0, 16, 3, 3, 5>(buf, nt, tw, tid); break;
+        case kPlan1080: fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, nt, tw, tid); break;
+        case kPlan1280: fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, nt, tw, tid); break;
+        case kPlan1920: fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, nt, tw, tid); break;
         default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid); break;
     }
 }

@@ -243,12 +243,15 @@ bool lenia::plan_make(int n, const int *radix, int ns, FftPlan &p) {
     p.nStages = ns;
     // the radix sequences with a compile-time kernel
     struct Fixed {
-        int n, ns, r[3];
+        int n, ns, r[5];
         uint32_t code;
     };
-    static const Fixed fixed[] = {{64, 2, {8, 8, 0}, kPlan64},        {128, 2, {16, 8, 0}, kPlan128},
-                                  {256, 2, {16, 16, 0}, kPlan256},    {512, 3, {8, 8, 8}, kPlan512},
-                                  {1024, 3, {16, 8, 8}, kPlan1024}, {2048, 3, {16, 16, 8}, kPlan2048}};
+    static const Fixed fixed[] = {
+        {64, 2, {8, 8}, kPlan64},          {128, 2, {16, 8}, kPlan128},         {256, 2, {16, 16}, kPlan256},
+        {512, 3, {8, 8, 8}, kPlan512},     {1024, 3, {16, 8, 8}, kPlan1024},    {2048, 3, {16, 16, 8}, kPlan2048},
+        {96, 3, {8, 4, 3}, kPlan96},       {192, 3, {8, 8, 3}, kPlan192},       {384, 3, {16, 8, 3}, kPlan384},
+        {720, 4, {16, 3, 3, 5}, kPlan720}, {1080, 5, {8, 3, 3, 3, 5}, kPlan1080}, {1280, 3, {16, 16, 5}, kPlan1280},
+        {1920, 4, {16, 8, 3, 5}, kPlan1920}};
     p.code = kPlanGeneric;
     for (const Fixed &f : fixed)
         if (f.n == n && f.ns == ns && std::equal(radix, radix + ns, f.r)) p.code = f.code;

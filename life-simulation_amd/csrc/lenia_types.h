@@ -59,8 +59,8 @@ struct FftStage {
 };
 
 // FFT plan of one length: stages in order
-// Radix sequences that also have a compile-time kernel (power-of-two lengths
-// with their default plans).  The fixed kernels run the same butterflies in
+// Radix sequences that also have a compile-time kernel (the default plans of
+// common lengths).  The fixed kernels run the same butterflies in
 // the same order as the table-driven one, so both give the same bits.
 enum : uint32_t {
     kPlanGeneric = 0,
@@ -70,6 +70,13 @@ enum : uint32_t {
     kPlan512 = 4,   // 8 8 8
     kPlan1024 = 5,  // 16 8 8
     kPlan2048 = 6,  // 16 16 8
+    kPlan96 = 7,    // 8 4 3
+    kPlan192 = 8,   // 8 8 3
+    kPlan384 = 9,   // 16 8 3
+    kPlan720 = 10,  // 16 3 3 5
+    kPlan1080 = 11, // 8 3 3 3 5
+    kPlan1280 = 12, // 16 16 5
+    kPlan1920 = 13, // 16 8 3 5
 };
 
 struct FftPlan {
