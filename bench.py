@@ -60,11 +60,24 @@ def cpu_baseline(desc: dict, budget_s: float = 12.0) -> dict:
         if time.perf_counter() - t0 >= budget_s:
             break
     dt = time.perf_counter() - t0
+    # the same step on one core (a shorter sample)
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    n1 = 0
+    while True:
+        A = O.step(desc, A, use_fft=True)
+        n1 += 1
+        if time.perf_counter() - t1 >= budget_s / 3:
+            break
+    dt1 = time.perf_counter() - t1
+    O.set_threads(threads)
     return {"value": round(W * H * n / dt / 1e6, 3), "unit": "Mcell-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"FP64 oracle FFT step (oracle/lenia_oracle.c), {n} steps of the same "
                       f"{W}x{H} K={len(desc['kernels'])} world after 1 warm-up step, "
-                      f"OpenMP {threads} threads, {dt:.1f} s"}
+                      f"OpenMP {threads} threads, {dt:.1f} s",
+            "one_core": {"value": round(W * H * n1 / dt1 / 1e6, 3), "cores": 1,
+                         "sample": f"{n1} steps on 1 thread, {dt1:.1f} s"}}
 
 
 def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int = 20) -> dict:
