@@ -207,13 +207,29 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t g, uint32_t m) {
     return m ? __umulhi(g, m) : g;
 }
 
-// LDS swizzle of transform buffers: within every 32-element (256-byte) row of
-// the LDS banks the low index bits are XORed with the row number.  Stockham
-// stores land at stride R (up to 16 x 8 bytes); unswizzled, 32 of the 64 lanes
-// hit one bank.  With the swizzle every store and load of a stage is at most
-// 2-way (the minimum for 8-byte accesses).  A bijection within each row, so
-// buffers keep their size.
-__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 5) & 31); }
+// LDS placement of transform buffers: entry i sits at i + i/32, one unused
+// entry after every 32 (256 bytes, one row of the LDS banks).  Stockham stores
+// land at stride R (up to 32 x 8 bytes); unpadded, 32 of the 64 lanes hit one
+// bank, padded every store and load of a stage is at most 2-way (the minimum
+// for 8-byte accesses).  Unlike an XOR swizzle the placement is affine on
+// each run that does not cross a 32-entry row, so the R accesses of a
+// butterfly are one base address plus compile-time offsets (see stage_off).
+// A region of n entries spans lds_span(n) entries.
+__device__ __forceinline__ int sw(int i) { return i + (i >> 5); }
+
+// sw(base + r s) - sw(base) for the R accesses of a Stockham butterfly, when
+// that difference does not depend on base: s a multiple of 32, or s a divisor
+// of 32 with every butterfly's run inside one 32-entry row (N a multiple of
+// 32, and for stores the run of R s entries aligned to min(R s, 32)).
+template <int N, int R, int S, bool STORE>
+constexpr bool stage_fold() {
+    if (S % 32 == 0) return true;
+    if (N % 32 != 0 || 32 % S != 0) return false;
+    if (!STORE) return true;  // loads: base = b N + j with j < S
+    return (S * R) % 32 == 0 || 32 % (S * R) == 0;
+}
+template <int S>
+__device__ __forceinline__ constexpr int stage_off(int r) { return r * S + (r * S) / 32; }
 
 // One Stockham stage over nt transforms in LDS (swizzled).  Every thread
 // first loads all its butterflies, the group synchronises, then each thread
@@ -300,6 +316,8 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
     constexpr int MB = MAXV / R;
     static_assert(N % (NS * R) == 0, "radix sequence does not divide the length");
     if constexpr (MB > 0) {
+        constexpr bool kFoldLd = stage_fold<N, R, NR, false>();
+        constexpr bool kFoldSt = stage_fold<N, R, NS, true>();
         const int nb = nt * NR;
         float2 v[MB][R];
         float2 w[MB][R - 1];
@@ -309,8 +327,14 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
             if (g < nb) {
                 const int b = g / NR, j = g % NR, k = j % NS;
                 const int base = b * N + j;
+                if constexpr (kFoldLd) {
+                    const float2 *src = buf + sw(base);
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[i][r] = buf[sw(base + r * NR)];
+                    for (int r = 0; r < R; ++r) v[i][r] = src[stage_off<NR>(r)];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[i][r] = buf[sw(base + r * NR)];
+                }
                 const float2 *t = tw + TWOFF + k * (R - 1);
 #pragma unroll
                 for (int r = 0; r < R - 1; ++r) w[i][r] = t[r];
@@ -328,8 +352,14 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
                 }
                 Dft<R, INV>::run(v[i]);
                 const int base = b * N + (j - k) * R + k;
+                if constexpr (kFoldSt) {
+                    float2 *dst = buf + sw(base);
 #pragma unroll
-                for (int r = 0; r < R; ++r) buf[sw(base + r * NS)] = v[i][r];
+                    for (int r = 0; r < R; ++r) dst[stage_off<NS>(r)] = v[i][r];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r) buf[sw(base + r * NS)] = v[i][r];
+                }
             }
         }
         __syncthreads();

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
     const size_t sStride = (size_t)Wh * H;
     float2 *tw = smem;
     float2 *Z = smem + W;
-    float *red = reinterpret_cast<float *>(Z + ((K * W + 31) & ~31));
+    float *red = reinterpret_cast<float *>(Z + lds_span(K * W));
     load_twiddles(tw, a.twW, W, tid, T);
     const GrowthParams &gr = *a.gDev;
     const float dt = gr.dt;
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
     }
     __syncthreads();
     fft_run<false, kAV, T>(X, a.planH, K, tw, tid);
-    float2 *Y = X + ((K * H + 31) & ~31);
+    float2 *Y = X + lds_span(K * H);
     // shell partial sums: thread-strided over ky, then a fixed-order LDS tree
     const float wt = (c == 0 || c == Wh - 1) ? 1.0f : 2.0f;
     const int nb = a.nShells;  // total (k, j) pairs, <= 16
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
         Y[sw(e)] = y;
     }
     // reduce the partial sums across the workgroup (the LDS after Y)
-    float *red = reinterpret_cast<float *>(Y + ((H + 31) & ~31));
+    float *red = reinterpret_cast<float *>(Y + lds_span(H));
     for (int i = 0; i < nb; ++i) {
         red[tid] = acc[i] * wt;
         __syncthreads();
@@ -299,23 +299,21 @@ static hipError_t by_threads(int T, F f) {
     return hipGetLastError();
 }
 
-// LDS: the twiddles, then the swizzled regions rounded up to 32 entries (sw()
-// permutes inside aligned blocks of 32)
-static inline int r32(int n) { return (n + 31) & ~31; }
+// LDS: the twiddles, then the transform regions (lds_span entries each)
 
 size_t adj_lds_rows(const AdjArgs &a, int T) {
     // mid: twiddles | Z [K][W] | reduction [T];  edge: twiddles | Z [W]
-    return (size_t)(a.W + r32(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
+    return (size_t)(a.W + lds_span(a.K * a.W)) * sizeof(float2) + (size_t)T * sizeof(float);
 }
 
 size_t adj_lds_cols(const AdjArgs &a, int T) {
     // twiddles | X [K][H] | Y [H] | reduction [T]
-    return (size_t)(a.H + r32(a.K * a.H) + r32(a.H)) * sizeof(float2) + (size_t)T * sizeof(float);
+    return (size_t)(a.H + lds_span(a.K * a.H) + lds_span(a.H)) * sizeof(float2) + (size_t)T * sizeof(float);
 }
 
 hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
                            hipStream_t s) {
-    const size_t lds = (size_t)(a.W + r32(a.W)) * sizeof(float2);
+    const size_t lds = (size_t)(a.W + lds_span(a.W)) * sizeof(float2);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
         hipLaunchKernelGGL(k_adj_edge<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, G1, gpb, gOut);

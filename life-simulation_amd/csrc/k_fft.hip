@@ -381,7 +381,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             }
         }
         if (KPAR) {
-            float2 *Y = X + ((ne + 31) & ~31);  // own 32-aligned swizzle rows
+            float2 *Y = X + lds_span(ne);
             const int nk = K * ne;
             float sr[kMaxV];
 #pragma unroll

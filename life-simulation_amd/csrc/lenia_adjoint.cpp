@@ -220,18 +220,18 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
         // workgroup sizes: about 2 values per thread for the K transforms of
         // a row pair or column -- short stages over many threads, since one
         // reverse step has only H/2 + W/2+1 workgroups -- within <= 16 values
-        // per thread and 64 KB of LDS
+        // per thread and the workgroup's LDS
         const int mv = fft_max_values();
         auto pick = [&](const FftPlan &p, int nt, bool rows) {
             int want = 64;
             while (want < 512 && want * 2 < nt * p.n) want *= 2;
             for (int T2 = want; T2 <= 512; T2 *= 2)
                 if (plan_fits(p, nt, T2, mv) && plan_fits(p, 1, T2, mv) &&
-                    (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= 65536)
+                    (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= kFftLdsMax)
                     return T2;
             for (int T2 = want / 2; T2 >= 64; T2 /= 2)
                 if (plan_fits(p, nt, T2, mv) && plan_fits(p, 1, T2, mv) &&
-                    (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= 65536)
+                    (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= kFftLdsMax)
                     return T2;
             return 0;
         };

@@ -41,7 +41,7 @@ int lenia::hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
-// dynamic LDS a workgroup of this library may request
+// dynamic LDS of the direct forms' workgroups
 static constexpr size_t kLdsMax = 65536;
 
 extern "C" const char *lenia_last_error(void) { return g_err.c_str(); }
@@ -507,24 +507,23 @@ static int build_kernels_fft(lenia_world *w) {
 // (kpar, K transforms per tile slot), and the radix plan.  Explicit values
 // from the descriptor win.  Otherwise: the smallest tile (nt = 1) with the
 // smallest workgroup that holds <= 16 values per thread, side by side when it
-// fits in 64 KB of LDS.
+// fits in the workgroup's LDS.
 
-// The swizzle sw(i) permutes indices inside aligned blocks of 32, so every
-// swizzled region is sized up to a multiple of 32 entries.
-static inline int round32(int n) { return (n + 31) & ~31; }
+// Transform regions span lds_span(n) entries (one unused entry after every
+// 32, the LDS placement of the kernels).
 size_t lenia::lds_rows(int W, int P, int K, bool kpar) {
-    return (size_t)(W + round32((kpar ? K : 1) * P * W)) * sizeof(float2);
+    return (size_t)(W + lds_span((kpar ? K : 1) * P * W)) * sizeof(float2);
 }
 size_t lenia::lds_cols(int H, int C, int K, bool kpar) {
-    return (size_t)(H + round32(C * H) + (kpar ? round32(K * C * H) : 0)) * sizeof(float2);
+    return (size_t)(H + lds_span(C * H) + (kpar ? lds_span(K * C * H) : 0)) * sizeof(float2);
 }
 
 // does (nt transforms per tile, T threads, side-by-side kernels or not) fit
-// this plan, the per-thread register budget and 64 KB of LDS?
+// this plan, the per-thread register budget and the workgroup's LDS?
 bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar) {
     const int mv = fft_max_values();
     const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
-    if (lds > kLdsMax) return false;
+    if (lds > kFftLdsMax) return false;
     return plan_fits(p, kpar ? nt * K : nt, T, mv) && plan_fits(p, nt, T, mv);
 }
 

@@ -10,6 +10,10 @@ namespace lenia {
 constexpr int kMaxKernels = 4;
 constexpr int kMaxStages = 8;  // FFT plan: up to 8 radix stages
 
+// LDS entries that a transform region of n complex values spans: the
+// placement i -> i + i/32 leaves one unused entry after every 32
+constexpr int lds_span(int n) { return n + (n + 31) / 32; }
+
 // cosine-palette colour map (ColorMapParams, LifeCore/Render/ColorMapPass.h:16-25;
 // same layout as lenia_colormap in the C ABI)
 struct ColorMapParams {
