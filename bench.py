@@ -87,6 +87,9 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
     import lenia
     from lenia_cases import sweep_worlds
 
+    graph_steps = min(graph_steps, steps)
+    if steps % graph_steps:
+        raise SystemExit(f"bench.py: sweep steps ({steps}) must be a multiple of {graph_steps}")
     descs = sweep_worlds(256)
     worlds = [lenia.World(ctx, d) for d in descs]
     try:
@@ -102,23 +105,25 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
             ctx.sync()
             b.step(steps, graph_steps)
             batch_ms = ctx.last_gpu_ms()
-        # plain: each world alone, one after the other
-        for w in worlds:
-            w.step(2)
-        ctx.sync()
-        t0 = time.perf_counter()
-        for w in worlds:
-            w.step(plain_steps)
-        ctx.sync()
-        plain_s = time.perf_counter() - t0
         batch_rate = cells * steps / (batch_ms * 1e-3) / 1e6
-        plain_rate = cells * plain_steps / plain_s / 1e6
+        plain_rate = None
+        if plain_steps:
+            # plain: each world alone, one after the other
+            for w in worlds:
+                w.step(2)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for w in worlds:
+                w.step(plain_steps)
+            ctx.sync()
+            plain_s = time.perf_counter() - t0
+            plain_rate = cells * plain_steps / plain_s / 1e6
         gbs = bytes_step * steps / (batch_ms * 1e-3) / 1e9
         return {"worlds": len(descs), "cells": cells, "steps": steps,
                 "graph_steps": graph_steps, "ms_per_step": round(batch_ms / steps, 5),
                 "mcell_steps_per_s": round(batch_rate, 1),
-                "plain_mcell_steps_per_s": round(plain_rate, 1),
-                "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+                "plain_mcell_steps_per_s": round(plain_rate, 1) if plain_rate else None,
+                "speedup_vs_plain": round(batch_rate / plain_rate, 2) if plain_rate else None,
                 "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5)}
     finally:
@@ -270,6 +275,11 @@ def main() -> int:
                          "(row threads, row pairs per tile, column threads, columns per tile)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain fp32 streaming kernel over MIB MiB (counter scaling)")
+    ap.add_argument("--only", choices=["sweep", "adjoint", "direct"], default="",
+                    help="run one side leg alone and print its JSON object (for profiling)")
+    ap.add_argument("--sweep-steps", type=int, default=1000, help="timed batched sweep steps")
+    ap.add_argument("--plain-steps", type=int, default=20,
+                    help="steps per world of the one-world-per-launch loop (0: skip it)")
     args = ap.parse_args()
     if args.copy_calibration:
         print(json.dumps(copy_calibration(args.copy_calibration)))
@@ -279,6 +289,13 @@ def main() -> int:
               "(worlds shard across GPUs with no communication; see the sharding notes)",
               file=sys.stderr)
         return 2
+    if args.only:
+        import lenia
+        legs = {"sweep": lambda c: sweep_bench(c, args.sweep_steps, plain_steps=args.plain_steps),
+                "adjoint": adjoint_bench, "direct": direct_forms_bench}
+        with lenia.Context(0) as ctx:
+            print(json.dumps({args.only: legs[args.only](ctx)}))
+        return 0
 
     import lenia
     from lenia_cases import config1
@@ -305,7 +322,8 @@ def main() -> int:
         passes = w.pass_times(min(args.steps, 500))
         tiles = w.resolved()
         A = w.read()
-        batched = None if args.no_sweep else sweep_bench(ctx)
+        batched = None if args.no_sweep else sweep_bench(ctx, args.sweep_steps,
+                                                          plain_steps=args.plain_steps)
         adjoint = None if args.no_adjoint else adjoint_bench(ctx)
         direct = None if args.no_direct else direct_forms_bench(ctx)
     sec = gpu_ms / 1e3

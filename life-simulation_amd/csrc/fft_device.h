@@ -231,7 +231,7 @@ constexpr bool stage_fold() {
 template <int S>
 __device__ __forceinline__ constexpr int stage_off(int r) { return r * S + (r * S) / 32; }
 
-// One Stockham stage over nt transforms in LDS (swizzled).  Every thread
+// One Stockham stage over nt transforms in LDS (padded placement).  Every thread
 // first loads all its butterflies, the group synchronises, then each thread
 // twiddles, transforms and stores, so the stage runs in place in one LDS
 // buffer.  `tws` is this stage's twiddle table, [k][r-1] for k < Ns,

@@ -46,7 +46,7 @@ __device__ __forceinline__ void load_twiddles(float2 *tw, const float *src, int 
 
 // half spectrum slot (float4 per kx: rows 2rp and 2rp+1) -> packed complex row
 // Z[base + x] = a[x] + i b[x] (Hermitian mirror fills the negative
-// frequencies; conjugated when CONJ).  LDS indices are swizzled as a whole:
+// frequencies; conjugated when CONJ).  LDS indices are placed as a whole:
 // slot s of an FFT batch lives at sw(s W + x), not at s W + sw(x).
 template <bool CONJ>
 __device__ __forceinline__ void gather_row_pair(float2 *Z, int base, const float *G, int W, int H2,
