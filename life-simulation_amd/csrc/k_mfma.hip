@@ -141,31 +141,77 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         const StepClock ck = clock_at(wd.clock, stepOff);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
-        // state tile: rows y0-h .. y0+32NB-1+h, columns x0-h .. x0-h+cols-1
-        // (toroidal); kStage loads in flight per thread before any is stored
-        const int nst = rows * cols;
-        for (int base = 0; base < nst; base += kMT * kStage) {
-            float v[kStage];
-            int at[kStage];
+        // state tile: rows y0-h .. y0+32NB-1+h, columns xs .. xs+cols-1 with
+        // xs = x0-h-shift (toroidal); up to kStage floats in flight per
+        // thread before any is stored
+        const int xs = x0 - h - wd.shift;
+        if (wd.vec4) {
+            // W and xs multiples of 4: aligned 16-byte loads that never
+            // straddle the wrap; rows and column groups wrap by one period
+            // except near a field smaller than the tile (then by modulo)
+            constexpr int G4 = 4 * KS;  // float4 groups per staged row
+            constexpr int kStage4 = kStage / 4;
+            const int ngr = rows * G4;
+            for (int base = 0; base < ngr; base += kMT * kStage4) {
+                float4 v[kStage4];
+                int at[kStage4];
 #pragma unroll
-            for (int u = 0; u < kStage; ++u) {
-                const int i = base + u * kMT + tid;
-                at[u] = -1;
-                if (i < nst) {
-                    const int ly = i / cols, lx = i - ly * cols;
-                    int gy = (y0 - h + ly) % H, gx = (x0 - h + lx) % W;
-                    if (gy < 0) gy += H;
-                    if (gx < 0) gx += W;
-                    v[u] = src[(size_t)gy * W + gx];
-                    at[u] = ly * pitch + lx;
+                for (int u = 0; u < kStage4; ++u) {
+                    const int i = base + u * kMT + tid;
+                    at[u] = -1;
+                    if (i < ngr) {
+                        const int ly = i / G4, q = i - ly * G4;
+                        int gy = y0 - h + ly, gx = xs + 4 * q;
+                        if (gy < 0 || gy >= H) gy = (gy % H + H) % H;
+                        if (gx < 0 || gx >= W) gx = (gx % W + W) % W;
+                        v[u] = gld4(reinterpret_cast<const LENIA_GLOBAL float4 *>(src + (size_t)gy * W + gx));
+                        at[u] = ly * pitch + 4 * q;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kStage4; ++u) {
+                    if (at[u] >= 0) {
+                        const float4 f = v[u];
+                        const unsigned short b0 = bf16_bits(f.x), b1 = bf16_bits(f.y);
+                        const unsigned short b2 = bf16_bits(f.z), b3 = bf16_bits(f.w);
+                        *reinterpret_cast<uint2 *>(tHi + at[u]) =
+                            make_uint2(b0 | ((uint32_t)b1 << 16), b2 | ((uint32_t)b3 << 16));
+                        if constexpr (SPLIT) {
+                            const unsigned short l0 = bf16_bits(f.x - bf16_value(b0));
+                            const unsigned short l1 = bf16_bits(f.y - bf16_value(b1));
+                            const unsigned short l2 = bf16_bits(f.z - bf16_value(b2));
+                            const unsigned short l3 = bf16_bits(f.w - bf16_value(b3));
+                            *reinterpret_cast<uint2 *>(tLo + at[u]) =
+                                make_uint2(l0 | ((uint32_t)l1 << 16), l2 | ((uint32_t)l3 << 16));
+                        }
+                    }
                 }
             }
+        } else {
+            const int nst = rows * cols;
+            for (int base = 0; base < nst; base += kMT * kStage) {
+                float v[kStage];
+                int at[kStage];
 #pragma unroll
-            for (int u = 0; u < kStage; ++u) {
-                if (at[u] >= 0) {
-                    const unsigned short hb = bf16_bits(v[u]);
-                    tHi[at[u]] = hb;
-                    if constexpr (SPLIT) tLo[at[u]] = bf16_bits(v[u] - bf16_value(hb));
+                for (int u = 0; u < kStage; ++u) {
+                    const int i = base + u * kMT + tid;
+                    at[u] = -1;
+                    if (i < nst) {
+                        const int ly = i / cols, lx = i - ly * cols;
+                        int gy = (y0 - h + ly) % H, gx = (xs + lx) % W;
+                        if (gy < 0) gy += H;
+                        if (gx < 0) gx += W;
+                        v[u] = src[(size_t)gy * W + gx];
+                        at[u] = ly * pitch + lx;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kStage; ++u) {
+                    if (at[u] >= 0) {
+                        const unsigned short hb = bf16_bits(v[u]);
+                        tHi[at[u]] = hb;
+                        if constexpr (SPLIT) tLo[at[u]] = bf16_bits(v[u] - bf16_value(hb));
+                    }
                 }
             }
         }
@@ -181,6 +227,16 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         // wave b < NB finishes output block b: its rows y0 + 32 b + n, columns
         // x0 + 8 (r/4) + 4 grp + (r%4) of lane slot r
         const int yb = y0 + 32 * wave + n;
+        // the owners' current state values, loaded now so that they arrive
+        // while the window rows multiply
+        float aCur[16];
+        if (MODE == 0 && wave < NB) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
+                aCur[r] = (x < W && yb < H) ? src[(size_t)yb * W + x] : 0.0f;
+            }
+        }
         float gacc[16];
 #pragma unroll
         for (int r = 0; r < 16; ++r) gacc[r] = 0.0f;
@@ -245,10 +301,8 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
-                if (x < W && yb < H) {
-                    float a = src[(size_t)yb * W + x];
-                    dst[(size_t)yb * W + x] = growth_finish(g, a, gacc[r], x, yb, W, H, seed, ck.frame);
-                }
+                if (x < W && yb < H)
+                    dst[(size_t)yb * W + x] = growth_finish(g, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
             }
         }
         __syncthreads();
@@ -300,8 +354,17 @@ void mfma_tile_shape(bool split, int *tx, int *ty) {
 
 int mfma_max_ksteps() { return kMaxSteps; }
 
-size_t mfma_lds_bytes(int h, bool split) {
-    const int ks = (32 + 2 * h + 15) / 16;
+// columns between the staged tile's first column and x0 - h: the tile then
+// starts on a multiple of 4 (x0 is a multiple of 32), for 16-byte loads
+int mfma_shift(int h, int W) { return W % 4 == 0 ? (4 - h % 4) % 4 : 0; }
+
+// 16-wide k-steps per window row: the staged columns j < 16 ks must cover
+// 32 + 2h + shift (the shift never adds a k-step: 32 + 2h is even and
+// shift <= 1 when 2h = 14 mod 16)
+int mfma_ksteps(int h, int W) { return (32 + 2 * h + mfma_shift(h, W) + 15) / 16; }
+
+size_t mfma_lds_bytes(int h, int W, bool split) {
+    const int ks = mfma_ksteps(h, W);
     const int pitch = 16 * ks + 8;
     const int rows = 32 * (split ? blocks_y<true>() : blocks_y<false>()) + 2 * h;
     return (size_t)rows * pitch * sizeof(unsigned short) * (split ? 2 : 1) +

@@ -390,13 +390,16 @@ static float bf16_value_host(uint16_t b) {
 
 // MFMA forms: Toeplitz fragments of every window row in MFMA register order.
 // For kernel k, window row dy, k-step s and lane l (output column m = l % 32,
-// k-group g = l / 32) the 8 values K[dy][16 s + 8 g + i - m], i < 8 (zero
-// outside the window), as three BF16 planes: hi = bf16(v), mid = bf16(v - hi),
-// lo = bf16(v - hi - mid).
+// k-group g = l / 32) the 8 values K[dy][16 s + 8 g + i - m - shift], i < 8
+// (zero outside the window), as three BF16 planes: hi = bf16(v), mid =
+// bf16(v - hi), lo = bf16(v - hi - mid).  The staged state tile starts shift
+// columns left of x0 - h (mfma_shift), so that it loads four aligned columns
+// at a time.
 static int build_toeplitz(lenia_world *w, const std::vector<float> &all, int h) {
     lenia_ctx *ctx = w->ctx;
     const int K = (int)w->desc.nKernels, n = 2 * h + 1;
-    const int ks = (32 + 2 * h + 15) / 16;
+    const int shift = mfma_shift(h, w->desc.width);
+    const int ks = mfma_ksteps(h, w->desc.width);
     w->ksteps = ks;
     const size_t nfr = (size_t)K * n * ks * 64 * 8;
     std::vector<uint16_t> hi(nfr), mid(nfr), lo(nfr);
@@ -405,7 +408,7 @@ static int build_toeplitz(lenia_world *w, const std::vector<float> &all, int h) 
             for (int s = 0; s < ks; ++s)
                 for (int l = 0; l < 64; ++l)
                     for (int i = 0; i < 8; ++i) {
-                        const int t = 16 * s + 8 * (l / 32) + i - (l % 32);
+                        const int t = 16 * s + 8 * (l / 32) + i - (l % 32) - shift;
                         const float v = (t >= 0 && t < n) ? all[((size_t)k * n + dy) * n + t] : 0.0f;
                         const size_t idx = ((((size_t)k * n + dy) * ks + s) * 64 + l) * 8 + i;
                         hi[idx] = bf16_bits_host(v);
@@ -645,6 +648,8 @@ int lenia::world_upload_desc(lenia_world *w) {
         md.ksteps = w->ksteps;
         md.tileCols = 16 * w->ksteps;
         md.pitch = md.tileCols + 8;
+        md.vec4 = md.W % 4 == 0;
+        md.shift = mfma_shift(w->half, md.W);
         md.g = w->g;
         md.toepHi = w->toepHi;
         md.toepMid = w->toepMid;
@@ -660,7 +665,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
                                hipMemcpyHostToDevice, ctx->stream), "upload tiles");
         w->nRowTiles = (int)t.size();
-        w->ldsRows = mfma_lds_bytes(w->half, w->split());
+        w->ldsRows = mfma_lds_bytes(w->half, md.W, w->split());
     } else {
         DirectWorldDev dd;
         std::memset(&dd, 0, sizeof(dd));
@@ -736,8 +741,8 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
     if (w->isMfma) {
         int h = 0;
         for (uint32_t k = 0; k < desc->nKernels; ++k) h = std::max(h, kernel_half(desc->kernels[k]));
-        const int ks = (32 + 2 * h + 15) / 16;
-        const size_t lds = mfma_lds_bytes(h, w->split());
+        const int ks = mfma_ksteps(h, W);
+        const size_t lds = mfma_lds_bytes(h, W, w->split());
         if (ks > mfma_max_ksteps() || lds > kLdsMax) {
             delete w;
             return fail(LENIA_ERR_UNSUPPORTED,

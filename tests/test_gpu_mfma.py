@@ -47,6 +47,8 @@ CASES = [
     ("ring3_R7", 96, 60, [ring(6, [1], .15, .015, .5), ring(7, [1, .5], .26, .036, .3),
                           ring(5.5, [.5, 1, .25], .12, .012, .2)]),
     ("exp_R32", 160, 100, [exp_shell(32)]),
+    # width not a multiple of 4: the state tile is staged one column per load
+    ("exp_R9_width198", 198, 70, [exp_shell(9)]),
 ]
 
 
