@@ -46,7 +46,8 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMW = 4;         // waves per workgroup
 constexpr int kMT = 64 * kMW;  // threads
 constexpr int kMaxSteps = 8;   // k-steps of 16 per window row (J <= 128, h <= 48)
-constexpr int kStage = 16;     // state-tile loads in flight per thread
+constexpr int kStage = 16;     // state-tile loads in flight per thread (one column each)
+constexpr int kStage4 = 8;     // the same, four columns each
 constexpr int kRedFloats = 2 * 64 * 16;  // reduction buffer: two waves' block totals
 
 // output blocks (32 rows each) per workgroup
@@ -142,7 +143,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
         // state tile: rows y0-h .. y0+32NB-1+h, columns xs .. xs+cols-1 with
-        // xs = x0-h-shift (toroidal); up to kStage floats in flight per
+        // xs = x0-h-shift (toroidal); kStage4 (kStage) loads in flight per
         // thread before any is stored
         const int xs = x0 - h - wd.shift;
         if (wd.vec4) {
@@ -150,7 +151,6 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             // straddle the wrap; rows and column groups wrap by one period
             // except near a field smaller than the tile (then by modulo)
             constexpr int G4 = 4 * KS;  // float4 groups per staged row
-            constexpr int kStage4 = kStage / 4;
             const int ngr = rows * G4;
             for (int base = 0; base < ngr; base += kMT * kStage4) {
                 float4 v[kStage4];
