@@ -120,7 +120,6 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         for (int i = 0; i < kDOX; ++i) gacc[i] = 0.0f;
         const int kBeg = MODE == 1 ? kSel : 0;
         const int kEnd = MODE == 1 ? kSel + 1 : (int)g.nKernels;
-        const int chunks = np / 8;
         const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
         const int pitch4 = pitch / 4;
         for (int k = kBeg; k < kEnd; ++k) {
@@ -128,16 +127,19 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
             float tot[kDOX];
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
+            const LENIA_CONST int *rc = (const LENIA_CONST int *)(wd.rowChunks) + (size_t)2 * k * n;
             for (int iy = 0; iy < n; ++iy) {
                 const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
                 const LENIA_CONST cv4 *wrow = wk + (size_t)iy * (np / 4);
+                // chunks [c0, c1) of this row hold its non-zero weights
+                const int c1 = rc[2 * iy + 1];
                 float rs[kDOX];
 #pragma unroll
                 for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
                 float va[8], vb[8], w[8];
-                lds_read8(trow, va);
-                int c = 0;
-                for (; c + 1 < chunks; c += 2) {
+                int c = rc[2 * iy];
+                lds_read8(trow + 2 * c, va);
+                for (; c + 1 < c1; c += 2) {
                     lds_read8(trow + 2 * c + 2, vb);
                     const_read8(wrow + 2 * c, w);
                     taps8(rs, w, va, vb);
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
                     const_read8(wrow + 2 * c + 2, w);
                     taps8(rs, w, vb, va);
                 }
-                if (c < chunks) {
+                if (c < c1) {
                     lds_read8(trow + 2 * c + 2, vb);
                     const_read8(wrow + 2 * c, w);
                     taps8(rs, w, va, vb);

@@ -453,11 +453,28 @@ static int build_kernels_direct(lenia_world *w) {
         for (size_t i = 0; i < win.size(); ++i) all[(size_t)k * n * n + i] = (float)win[i];
     }
     if (w->isMfma) return build_toeplitz(w, all, h);
-    // FP32 direct form: window rows zero padded to a multiple of 8 taps
+    // FP32 direct form: window rows zero padded to a multiple of 8 taps,
+    // followed by each row's range of 8-tap chunks holding a non-zero weight
+    // (the kernel skips the all-zero chunks at both ends of a row: adding
+    // w = 0 times a finite state value leaves a sum unchanged, so the bits
+    // do not change)
     const int np = direct_npad(h);
-    std::vector<float> padded((size_t)d.nKernels * n * np, 0.0f);
-    for (size_t r = 0; r < (size_t)d.nKernels * n; ++r)
-        for (int x = 0; x < n; ++x) padded[r * np + x] = all[r * n + x];
+    const size_t nrow = (size_t)d.nKernels * n;
+    std::vector<float> padded(nrow * np + 2 * nrow, 0.0f);
+    int32_t *chunks = reinterpret_cast<int32_t *>(padded.data() + nrow * np);
+    for (size_t r = 0; r < nrow; ++r) {
+        int lo = np / 8, hi = 0;
+        for (int x = 0; x < n; ++x) {
+            padded[r * np + x] = all[r * n + x];
+            if (all[r * n + x] != 0.0f) {
+                lo = std::min(lo, x / 8);
+                hi = std::max(hi, x / 8 + 1);
+            }
+        }
+        if (hi == 0) lo = 0;  // an all-zero row: empty range
+        chunks[2 * r] = lo;
+        chunks[2 * r + 1] = hi;
+    }
     size_t bytes = padded.size() * sizeof(float);
     if (!w->winF || w->winBytes != bytes) {
         dev_free(ctx, w->winF);
@@ -679,6 +696,8 @@ int lenia::world_upload_desc(lenia_world *w) {
         dd.npad = direct_npad(w->half);
         dd.g = w->g;
         dd.win = w->winF;
+        dd.rowChunks = reinterpret_cast<const int2 *>(
+            w->winF + (size_t)w->desc.nKernels * (2 * w->half + 1) * dd.npad);
         dd.A0 = w->A0;
         dd.A1 = w->A1;
         dd.clock = w->clock;

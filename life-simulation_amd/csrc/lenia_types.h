@@ -131,6 +131,7 @@ struct DirectWorldDev {
     int32_t pad0[2];
     GrowthParams g;
     const float *win;   // [K][2 half + 1][npad] float windows (zero padded)
+    const int2 *rowChunks;  // [K][2 half + 1]: first and past-last 8-tap chunk with a non-zero weight
     float *A0, *A1;     // ping-pong states
     StepClock *clock;
 };
