@@ -107,6 +107,33 @@ __global__ __launch_bounds__(T) void k_adj_edge(AdjArgs a, const float *G1, cons
 // saved potentials U [K][H][W] and state A -> per cell gp (gpb), gu_k; row
 // forward of gu_k -> FG.  Partials of dt, mu_k, sigma_k, w_k are added to
 // accRows[rp][v] (FP64).
+// Workgroup sums of v[i] (i < n, n <= NV) in a fixed order, added to dst[i]
+// in FP64: a butterfly over the 64 lanes of each wave (partners add the same
+// two values, so every lane holds the same bits), then the T/64 wave sums in
+// wave order.  One barrier; `red` holds NV * T/64 floats.
+template <int T, int NV>
+__device__ __forceinline__ void block_sums(const float (&v)[NV], int n, float scale, float *red,
+                                           LENIA_GLOBAL double *dst, int tid) {
+    constexpr int NW = T / 64;
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        if (i < n) {
+            float x = v[i] * scale;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if (lane == 0) red[i * NW + wave] = x;
+        }
+    }
+    __syncthreads();
+    if (tid < n) {
+        float t = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += red[tid * NW + w];
+        dst[tid] += (double)t;
+    }
+}
+
 template <int T>
 __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const float *A,
                                                const float *g, float *gpb, float *FG,
@@ -182,17 +209,7 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
     fft_run<false, kAV, T>(Z, a.planW, K, tw, tid);
     for (int k = 0; k < K; ++k) split_row_pair(Z, k * W, FG + k * sStride * 2, W, H2, rp, tid, T);
     // fixed-order workgroup sums of the partials
-    const int nv = 1 + 3 * K;
-    for (int v = 0; v < nv; ++v) {
-        red[tid] = vals[v];
-        __syncthreads();
-        for (int s = T / 2; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) gp(accRows)[(size_t)rp * 16 + v] += (double)red[0];
-        __syncthreads();
-    }
+    block_sums<T, 13>(vals, 1 + 3 * K, 1.0f, red, gp(accRows) + (size_t)rp * 16, tid);
 }
 
 // ------------------------------------------------------------------ columns
@@ -246,18 +263,10 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
         }
         Y[sw(e)] = y;
     }
-    // reduce the partial sums across the workgroup (the LDS after Y)
+    // reduce the partial sums across the workgroup (the LDS after Y); the
+    // barrier inside also orders the Y stores before the inverse transform
     float *red = reinterpret_cast<float *>(Y + lds_span(H));
-    for (int i = 0; i < nb; ++i) {
-        red[tid] = acc[i] * wt;
-        __syncthreads();
-        for (int s = T / 2; s > 0; s >>= 1) {
-            if (tid < s) red[tid] += red[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) gp(accCols)[(size_t)c * 16 + i] += (double)red[0];
-        __syncthreads();
-    }
+    block_sums<T, 16>(acc, nb, wt, red, gp(accCols) + (size_t)c * 16, tid);
     fft_run<true, kAV, T>(Y, a.planH, 1, tw, tid);
     for (int e = tid; e < H; e += T) gst2(gp(reinterpret_cast<float2 *>(G)) + (size_t)c * H + e, Y[sw(e)]);
 }
