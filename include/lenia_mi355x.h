@@ -88,7 +88,9 @@ typedef struct lenia_world_desc {
     /* FFT launch overrides (0 = library choice).  Plans: radices packed 4
      * bits each, first stage in the lowest nibble, 0xF encodes 16 and 0xE
      * encodes 32.  Tiles: row pairs per row-pass workgroup, columns per
-     * column-pass workgroup, and the two workgroup sizes (64, 128 or 256). */
+     * column-pass workgroup, and the two workgroup sizes (64, 128, 256 or
+     * 512; the row pass also 1024 when its kernel transforms run side by
+     * side). */
     uint32_t fftPlanW, fftPlanH;
     uint32_t rowPairsPerTile, colsPerTile;
     uint32_t rowThreads, colThreads;

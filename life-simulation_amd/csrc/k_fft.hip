@@ -567,6 +567,11 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
         case 128: rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 256: rows_dispatch<256>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 512: rows_dispatch<512>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 1024:  // steps with side-by-side transforms only (the serial form needs more registers)
+            if (mode == 1 && !kpar) return hipErrorInvalidValue;
+            if (extras) rows_launch<1024, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+            else rows_launch<1024, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -541,6 +541,8 @@ size_t lenia::lds_cols(int H, int C, int K, bool kpar) {
 // does (nt transforms per tile, T threads, side-by-side kernels or not) fit
 // this plan, the per-thread register budget and the workgroup's LDS?
 bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar) {
+    // 1024-thread workgroups: the row pass with side-by-side transforms only
+    if (T > 512 && (!rows || !kpar)) return false;
     const int mv = fft_max_values();
     const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
     if (lds > kFftLdsMax) return false;
@@ -553,7 +555,7 @@ bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpa
 // ones only when no shape can run the plan.
 static bool choose_plan(int N, int K, uint32_t packed, FftPlan &p, bool rows) {
     auto any_shape = [&](const FftPlan &q) {
-        for (int T = 64; T <= 512; T *= 2)
+        for (int T = 64; T <= 1024; T *= 2)
             for (int kp = 0; kp < 2; ++kp)
                 if (shape_ok(q, rows, 1, K, T, kp != 0)) return true;
         return false;
@@ -574,7 +576,7 @@ bool lenia::pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t w
             if (wantNt ? nt != (int)wantNt : nt > 1) continue;
             if (nt > count) break;
             int best = 0;
-            for (int T = 64; T <= 512; T *= 2) {
+            for (int T = 64; T <= 1024; T *= 2) {
                 if (wantT && T != (int)wantT) continue;
                 if (!shape_ok(p, rows, nt, K, T, kp != 0)) continue;
                 best = T;

@@ -29,7 +29,7 @@ std::vector<Shape> candidates(const FftPlan &p, bool rows, int count, int K) {
     std::vector<Shape> out;
     for (int kp = 1; kp >= 0; --kp)
         for (int nt = 1; nt <= 8 && nt <= count; nt *= 2)
-            for (int T = 64; T <= 512; T *= 2)
+            for (int T = 64; T <= 1024; T *= 2)
                 if (shape_ok(p, rows, nt, K, T, kp != 0)) out.push_back({nt, T, kp != 0});
     return out;
 }
