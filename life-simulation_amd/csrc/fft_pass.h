@@ -1,0 +1,530 @@
+// FFT-convolution form of the Lenia step on gfx950.
+//
+// The reference runs, per substep and kernel count K, one dispatch per radix-2
+// stage and axis plus R<->C, multiply and growth passes: 84 passes for
+// 1024x512, K=3 (Modules/FieldModules/Lenia.cpp:576-643, encodeFFT2D :57-85).
+// Here a step is two kernels, each touching HBM once per operand:
+//
+//   column pass  reads the state's half-spectrum F (after the row FFT),
+//                transforms each column forward, multiplies by the K real
+//                kernel spectra and transforms back: writes G_k, k < K.
+//   row pass     for each row pair: K inverse complex FFTs of G_k packed as
+//                one complex row (two real rows per transform), growth and
+//                clip for every kernel (leniaGrowthMulti, Lenia.metal:161-232),
+//                in-place state update, then the forward real FFT of the new
+//                rows, which becomes the next step's F.
+//
+// Spectra are stored column-major ([column][row]) so the column pass reads and
+// writes whole contiguous columns and the row pass moves two adjacent rows
+// (16 bytes) per spectrum column; tiles are mapped so that neighbouring row
+// pairs run on one XCD and share its L2.  Every global operand of a tile is
+// requested before the FFT that precedes its use, so loads overlap the LDS
+// work.  The inverse 1/(W H) is folded into the kernel spectra, which are
+// computed in FP64 from the kernel window.
+//
+// This header holds the step kernels and their launch templates; each
+// k_fft_pass<T>.hip instantiates them for one workgroup size (separate
+// translation units compile in parallel), k_fft.hip holds the set-up kernels
+// and the entry points that pick the instantiation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_device.h"
+#include "lenia_device.h"
+#include "launch.h"
+
+namespace lenia {
+
+constexpr int kMaxV = 16;  // complex values per thread per stage
+
+__device__ __forceinline__ int tile_of(int ti, int nTiles) {
+    return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
+}
+
+// ---------------------------------------------------------------- row pass
+// Twiddle-table copy global -> LDS, split in two so its loads can be in
+// flight together with the other loads of a tile
+template <int T>
+struct TwCopy {
+    static constexpr int B = 8;
+    float2 v[B];
+    __device__ __forceinline__ void issue(const LENIA_GLOBAL float2 *src, int n, int tid) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            int t = tid + i * T;
+            if (t < n) v[i] = gld2(src + t);
+        }
+    }
+    __device__ __forceinline__ void store(float2 *dst, const LENIA_GLOBAL float2 *src, int n,
+                                          int tid) {
+#pragma unroll
+        for (int i = 0; i < B; ++i) {
+            int t = tid + i * T;
+            if (t < n) dst[t] = v[i];
+        }
+        for (int t = tid + B * T; t < n; t += T) dst[t] = gld2(src + t);
+    }
+};
+
+// Stored half-spectra of a tile's row pairs (kernels kBeg .. kBeg+kn-1) ->
+// full-length complex rows Z = Ga + i Gb, one LDS slot of nt rows per kernel,
+// with the Hermitian symmetry of each real row's spectrum for kx > W/2.
+// gather_issue puts the loads in flight; gather_store writes LDS.
+constexpr int kMG = kMaxV / 2 + 1;  // float4 loads per thread
+
+template <int T>
+__device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], const LENIA_GLOBAL float4 *G4,
+                                             size_t gStride, int kBeg, int kn, int H2, int nt,
+                                             int P, uint32_t mP, int nm, int tid) {
+    const int total = kn * nm;
+#pragma unroll
+    for (int i = 0; i < kMG; ++i) {
+        int m = tid + i * T;
+        if (m < total) {
+            int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
+            int mm = m - s * nm;
+            int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
+            if (p < nt) gv[i] = gld4(G4 + (size_t)(kBeg + s) * gStride + kx * H2 + p);
+        }
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[kMG], int kn, int W,
+                                             int nt, int P, uint32_t mP, int nm, int tid) {
+    const int total = kn * nm;
+#pragma unroll
+    for (int i = 0; i < kMG; ++i) {
+        int m = tid + i * T;
+        if (m < total) {
+            int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
+            int mm = m - s * nm;
+            int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
+            if (p < nt) {
+                float4 g4 = gv[i];
+                if (kx == 0 || kx == W / 2) {
+                    // real columns for a real field: drop the rounding
+                    // residue instead of leaking it across rows
+                    g4.y = 0.0f;
+                    g4.w = 0.0f;
+                }
+                const int base = (s * nt + p) * W;
+                Z[sw(base + kx)] = make_float2(g4.x - g4.w, g4.y + g4.z);
+                if (kx > 0 && kx < W / 2)
+                    Z[sw(base + W - kx)] = make_float2(g4.x + g4.w, g4.z - g4.y);
+            }
+        }
+    }
+}
+
+// forward FFT of the tile's (new) row pairs, split into the two real rows'
+// half-spectra Fa = (Z[k] + conj Z[W-k]) / 2, Fb = (Z[k] - conj Z[W-k]) / 2i
+template <int T>
+__device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, const float2 *tw,
+                                             int rp0, int nt, int tid) {
+    const int W = wd.W, H2 = wd.H / 2, P = wd.P;
+    const int nm = P * wd.Wh;
+    fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+    LENIA_GLOBAL float4 *F4 = gp(reinterpret_cast<float4 *>(wd.F)) + rp0;
+    for (int m = tid; m < nm; m += T) {
+        int kx = (int)fast_div((uint32_t)m, wd.mP), p = m - kx * P;
+        if (p >= nt) continue;
+        float2 z1 = Z[sw(p * W + kx)];
+        float2 z2 = Z[sw(p * W + (kx == 0 ? 0 : W - kx))];
+        gst4(F4 + kx * H2 + p, make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
+                                           0.5f * (z1.y + z2.y), 0.5f * (z2.x - z1.x)));
+    }
+}
+
+// MODE 0: forward only (state -> F); 1: step (G -> growth -> state -> F);
+// 2: potential of kernel kSel only (G_kSel -> out), no state change.
+// KPAR: the K inverse transforms of a tile run side by side (one LDS slot per
+// kernel, spread over the workgroup's waves); otherwise they run one after
+// the other with the growth sums kept in LDS.
+// EXTRAS: the noise / muJitter / inject terms are compiled in (with all three
+// amounts zero both variants compute the same bits).
+// LDS: twiddles [W] | KPAR: Z [K][nt][W] ; serial: Z [nt][W] (sums in registers)
+template <int MODE, int T, bool EXTRAS, bool KPAR>
+__global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles,
+                                                uint32_t stepOff, int kSel,
+                                                float *__restrict__ out) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
+    if (ti < nTiles) do {
+        const int2 tile = tiles[tile_of(ti, nTiles)];
+        const FftWorldDev &wd = worlds[tile.x];
+        const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
+        const int rp0 = tile.y * P;
+        const int nt = min(P, H2 - rp0);
+        const uint32_t mP = wd.mP;
+        float2 *tw = smem;
+        float2 *Z = smem + W;
+        LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
+        const int nq = nt * W;
+        const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
+        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twW));
+        TwCopy<T> twc;
+        twc.issue(twg, W, tid);
+
+        if (MODE == 0) {
+            twc.store(tw, twg, W, tid);
+            for (int q = tid; q < nq; q += T) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ia = 2 * p * W + x;
+                Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
+            }
+            __syncthreads();
+            forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+            __syncthreads();
+            continue;
+        }
+        const GrowthParams &g = wd.g;
+        const GrowthRegs gr = growth_regs(g);
+        LENIA_GLOBAL float *const saveU = gp(wd.saveU);
+        LENIA_GLOBAL float *const saveA = gp(wd.saveA);
+        const StepClock ck = clock_at(wd.clock, stepOff);
+        const uint32_t seed = clock_seed(ck, g.moduleSeed);
+        const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
+        const size_t gStride = (size_t)Wh * H2;
+        float ar0[kMaxV], ar1[kMaxV];
+        if (MODE == 1) {
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    ar0[i] = Arow[ia];
+                    ar1[i] = Arow[ia + W];
+                }
+            }
+        }
+        if (KPAR || MODE == 2) {
+            const int kBeg = MODE == 2 ? kSel : 0;
+            const int kn = MODE == 2 ? 1 : (int)g.nKernels;
+            float4 gv[kMG];
+            gather_issue<T>(gv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, tid);
+            twc.store(tw, twg, W, tid);
+            gather_store<T>(Z, gv, kn, W, nt, P, mP, nm, tid);
+            __syncthreads();
+            fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
+            if (MODE == 2) {
+                for (int q = tid; q < nq; q += T) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    float2 z = Z[sw(q)];
+                    LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
+                    o[ia] = z.x;
+                    o[ia + W] = z.y;
+                }
+                __syncthreads();
+                continue;
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ya = 2 * (rp0 + p);
+                    float ja = 0.0f, jb = 0.0f;
+                    if (EXTRAS) {
+                        ja = cell_jitter(g, x, ya, seed, ck.frame);
+                        jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
+                    }
+                    float2 z[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k < kn) z[k] = Z[sw(k * nq + q)];
+                    float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (k < kn) {
+                            sa += growth_term_r(gr, k, z[k].x, ja);
+                            sb += growth_term_r(gr, k, z[k].y, jb);
+                            if (saveU) {
+                                LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
+                                su[0] = z[k].x;
+                                su[W] = z[k].y;
+                            }
+                        }
+                    }
+                    if (saveA) {
+                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
+                        sa0[0] = ar0[i];
+                        sa0[W] = ar1[i];
+                    }
+                    if (EXTRAS) {
+                        ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
+                        ar1[i] = growth_finish(g, ar1[i], sb, x, ya + 1, W, H, seed, ck.frame);
+                    } else {
+                        ar0[i] = growth_plain_r(gr, ar0[i], sa);
+                        ar1[i] = growth_plain_r(gr, ar1[i], sb);
+                    }
+                }
+            }
+            // stores after all LDS reads (no read waits behind a store)
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ia = 2 * p * W + x;
+                    Arow[ia] = ar0[i];
+                    Arow[ia + W] = ar1[i];
+                    Z[sw(q)] = make_float2(ar0[i], ar1[i]);
+                }
+            }
+        } else {
+            float sa[kMaxV], sb[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) sa[i] = sb[i] = 0.0f;
+            float4 gv[kMG];
+            gather_issue<T>(gv, G4, gStride, 0, 1, H2, nt, P, mP, nm, tid);
+            twc.store(tw, twg, W, tid);
+            for (int k = 0; k < (int)g.nKernels; ++k) {
+                gather_store<T>(Z, gv, 1, W, nt, P, mP, nm, tid);
+                // the next kernel's spectrum loads fly during this transform
+                if (k + 1 < (int)g.nKernels)
+                    gather_issue<T>(gv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, tid);
+                __syncthreads();
+                // table-driven transform inside the runtime kernel loop: the
+                // compile-time one would keep its addresses live across it
+                fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+#pragma unroll
+                for (int i = 0; i < kMaxV; ++i) {
+                    int q = tid + i * T;
+                    if (q < nq) {
+                        int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                        int ya = 2 * (rp0 + p);
+                        float2 z = Z[sw(q)];
+                        float ja = 0.0f, jb = 0.0f;
+                        if (EXTRAS) {
+                            ja = cell_jitter(g, x, ya, seed, ck.frame);
+                            jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
+                        }
+                        sa[i] += growth_term_r(gr, k, z.x, ja);
+                        sb[i] += growth_term_r(gr, k, z.y, jb);
+                        if (saveU) {
+                            LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
+                            su[0] = z.x;
+                            su[W] = z.y;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int q = tid + i * T;
+                if (q < nq) {
+                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                    int ya = 2 * (rp0 + p);
+                    float a2, b2;
+                    if (saveA) {
+                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
+                        sa0[0] = ar0[i];
+                        sa0[W] = ar1[i];
+                    }
+                    if (EXTRAS) {
+                        a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
+                        b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
+                    } else {
+                        a2 = growth_plain_r(gr, ar0[i], sa[i]);
+                        b2 = growth_plain_r(gr, ar1[i], sb[i]);
+                    }
+                    int ia = 2 * p * W + x;
+                    Arow[ia] = a2;
+                    Arow[ia + W] = b2;
+                    Z[sw(q)] = make_float2(a2, b2);
+                }
+            }
+        }
+        __syncthreads();
+        forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+        __syncthreads();
+    } while (0);
+}
+
+// ---------------------------------------------------------------- column pass
+// LDS: twiddles [H] | X [C][H] | KPAR: Y [K][C][H]
+template <int T, bool KPAR>
+__global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles) {
+    extern __shared__ float2 smem[];
+    const int tid = threadIdx.x;
+    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
+    if (ti < nTiles) do {
+        const int2 tile = tiles[tile_of(ti, nTiles)];
+        const FftWorldDev &wd = worlds[tile.x];
+        const int H = wd.H, Wh = wd.Wh;
+        const int c0 = tile.y * wd.C;
+        const int nt = min(wd.C, Wh - c0);
+        const int ne = nt * H;
+        const int K = (int)wd.g.nKernels;
+        float2 *tw = smem;
+        float2 *X = smem + H;
+        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twH));
+        TwCopy<T> twc;
+        twc.issue(twg, H, tid);
+        const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + (size_t)c0 * H;
+        const LENIA_GLOBAL float *S0 = gp(wd.S) + (size_t)c0 * H;
+        const size_t sStride = (size_t)Wh * H;
+        LENIA_GLOBAL float2 *G0 = gp(reinterpret_cast<float2 *>(wd.G)) + (size_t)c0 * H;
+        {
+            float2 fv[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) fv[i] = gld2(F + e);
+            }
+            twc.store(tw, twg, H, tid);
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) X[sw(e)] = fv[i];
+            }
+        }
+        if (KPAR) {
+            float2 *Y = X + lds_span(ne);
+            const int nk = K * ne;
+            float sr[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) {
+                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                    sr[i] = S0[(size_t)k * sStride + c];
+                }
+            }
+            __syncthreads();
+            fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            if (wd.saveAhat) {
+                LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+                for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
+            }
+            // all reads of the forward spectrum first, then the scaled copies
+            float2 xv[kMaxV];
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) {
+                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                    xv[i] = X[sw(c)];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
+            }
+            __syncthreads();
+            fft_run<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
+            for (int e = tid; e < nk; e += T) {
+                int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                gst2(G0 + (size_t)k * sStride + c, Y[sw(e)]);
+            }
+            __syncthreads();
+            continue;
+        }
+        float sr[kMaxV];
+#pragma unroll
+        for (int i = 0; i < kMaxV; ++i) {
+            int e = tid + i * T;
+            if (e < ne) sr[i] = S0[e];
+        }
+        __syncthreads();
+        fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+        float2 xs[kMaxV];
+#pragma unroll
+        for (int i = 0; i < kMaxV; ++i) {
+            int e = tid + i * T;
+            xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
+        }
+        if (wd.saveAhat) {
+            LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) gst2(sv + e, xs[i]);
+            }
+        }
+        for (int k = 0; k < K; ++k) {
+#pragma unroll
+            for (int i = 0; i < kMaxV; ++i) {
+                int e = tid + i * T;
+                if (e < ne) X[sw(e)] = cscale(xs[i], sr[i]);
+            }
+            if (k + 1 < K) {
+                const LENIA_GLOBAL float *Sn = S0 + (size_t)(k + 1) * sStride;
+#pragma unroll
+                for (int i = 0; i < kMaxV; ++i) {
+                    int e = tid + i * T;
+                    if (e < ne) sr[i] = Sn[e];
+                }
+            }
+            __syncthreads();
+            fft_lds<true, kMaxV, T>(X, wd.planH, nt, tw, tid);
+            LENIA_GLOBAL float2 *G = G0 + (size_t)k * sStride;
+            for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
+            __syncthreads();
+        }
+    } while (0);
+}
+
+// ---------------------------------------------------------------- launch templates
+template <int T, bool EX, bool KP>
+void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                        int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
+                        hipStream_t s) {
+    if (mode == 0)
+        launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, stepOff, kSel, out);
+    else if (mode == 1)
+        launch_timed(k_fft_rows<1, T, EX, KP>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
+                     stepOff, kSel, out);
+    else
+        launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, stepOff, kSel, out);
+}
+
+template <int T>
+void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
+                          const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
+                          int kSel, float *out, hipStream_t s) {
+    if (extras) {
+        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+    } else {
+        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+    }
+}
+
+template <int T>
+void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                          int grid, size_t lds, hipStream_t s) {
+    if (kpar)
+        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+    else
+        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+}
+
+// per-workgroup-size entry points (k_fft_pass<T>.hip)
+#define LENIA_FFT_PASS_DECL(T)                                                                  \
+    void fft_rows_pass_##T(int mode, bool extras, bool kpar, const FftWorldDev *worlds,         \
+                           const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff, \
+                           int kSel, float *out, hipStream_t s);                                \
+    void fft_cols_pass_##T(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,  \
+                           int grid, size_t lds, hipStream_t s);
+LENIA_FFT_PASS_DECL(64)
+LENIA_FFT_PASS_DECL(128)
+LENIA_FFT_PASS_DECL(256)
+LENIA_FFT_PASS_DECL(512)
+void fft_rows_pass_1024(int mode, bool extras, const FftWorldDev *worlds, const int2 *tiles,
+                        int nTiles, int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
+                        hipStream_t s);
+
+}  // namespace lenia
