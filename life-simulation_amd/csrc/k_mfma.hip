@@ -17,6 +17,12 @@
 // precomputed on the host in register order (64 lanes x 8 bf16 per k-step)
 // and streamed from L2.
 //
+// The state tile (rows y0-h .. y0+32NB-1+h, 16 KS columns) is staged in LDS as
+// BF16, four columns per 16-byte load when W is a multiple of 4: the tile then
+// starts `shift` = (-h mod 4) columns left of x0-h, so every load is aligned
+// and none straddles the toroidal wrap, and the fragments carry the shift
+// (K[dy][j - m - shift]).
+//
 // Work split: a workgroup owns a 32 (x) x 32 NB (y) output tile (NB blocks
 // stacked in y).  Its 4 waves split the 2h+1 window rows into quarters; each
 // wave applies every fragment it loads to all NB blocks (NB independent
