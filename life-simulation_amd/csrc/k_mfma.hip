@@ -252,20 +252,23 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             f32x16 total[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) total[b] = f32x16{};
-            if (r0 < r1) {
-                // two fragment sets: row dy+1 loads while row dy multiplies
-                Frags<SPLIT, KS> fa, fb;
-                load_toeplitz<SPLIT, KS>(fa, wd, k, r0, lane);
-                int dy = r0;
-                for (; dy + 1 < r1; dy += 2) {
-                    load_toeplitz<SPLIT, KS>(fb, wd, k, dy + 1, lane);
-                    rows_mma<SPLIT, KS, NB>(total, fa, colHi + dy * pitch, colLo + dy * pitch, 32 * pitch);
-                    load_toeplitz<SPLIT, KS>(fa, wd, k, min(dy + 2, r1 - 1), lane);
-                    rows_mma<SPLIT, KS, NB>(total, fb, colHi + (dy + 1) * pitch,
-                                            colLo + (dy + 1) * pitch, 32 * pitch);
+            // kD fragment sets in flight: row dy + kD loads while row dy
+            // multiplies (one row's MFMAs take less time than an L2 load)
+            constexpr int kD = SPLIT ? 3 : 4;
+            Frags<SPLIT, KS> f[kD];
+#pragma unroll
+            for (int i = 0; i < kD; ++i)
+                if (r0 + i < r1) load_toeplitz<SPLIT, KS>(f[i], wd, k, r0 + i, lane);
+            for (int dy = r0; dy < r1; dy += kD) {
+#pragma unroll
+                for (int i = 0; i < kD; ++i) {
+                    const int row = dy + i;
+                    if (row < r1) {
+                        rows_mma<SPLIT, KS, NB>(total, f[i], colHi + row * pitch, colLo + row * pitch,
+                                                32 * pitch);
+                        if (row + kD < r1) load_toeplitz<SPLIT, KS>(f[i], wd, k, row + kD, lane);
+                    }
                 }
-                if (dy < r1)
-                    rows_mma<SPLIT, KS, NB>(total, fa, colHi + dy * pitch, colLo + dy * pitch, 32 * pitch);
             }
             // fixed-order reduction of block b into wave b over the window-row
             // quarters q = 0..3 (wave w holds quarter w): (q_b + q_b+2) + (q_b+1 + q_b+3)
