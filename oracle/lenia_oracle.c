@@ -15,7 +15,9 @@
  *   Shaders/Field/Lenia.metal:32-61        leniaValueNoise / leniaInit
  *   Shaders/Field/Lenia.metal:63-128       leniaStep (direct potential + growth)
  *   Shaders/Field/Lenia.metal:156-232      leniaBell / leniaGrowthMulti
+ *   Shaders/Field/Lenia.metal:242-303      leniaStampInit (organism stamps)
  *   Shaders/Common/FFT.metal:13-45         FFT sign/scale convention
+ *   Shaders/Render/ColorMap.metal:17-75    colorMapField / colorMapFieldScaled
  *
  * Pinning (each check is independent of the formula it checks):
  *   hashes      known answers computed from an independent transcription, and
@@ -31,6 +33,12 @@
  *   growth      uniform-field closed form A' = clip(c + dt(2 bell(c)-1));
  *               zero field stays zero; numpy golden steps
  *   gradient    FP64 central finite differences of lo_rollout
+ *   stamps      one stamp is the organism whole in one of the 8 dihedral
+ *               orientations (all 8 occur over seeds); unrotated it is the
+ *               organism as stored; overlapping stamps take the maximum
+ *   colour map  palette closed forms (t = 0, the mono ramp, clamps, bias /
+ *               scale / exposure, the channel rule); 2x downsampling = mean of
+ *               four texels, wrap at the border, constants stay constant
  *   value noise parity unpinned against the reference (needs Metal FP32);
  *               pinned only by range and lattice-continuity invariants
  */
