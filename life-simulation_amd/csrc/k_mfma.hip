@@ -99,21 +99,32 @@ __device__ __forceinline__ void load_toeplitz(Frags<SPLIT, KS> &f, const MfmaWor
 }
 
 // one window row applied to the NB blocks, accumulated into total[b];
-// rowHi/rowLo point at the row of block 0
+// rowHi/rowLo point at the row of block 0.  All state operands of the row are
+// read from LDS first and the MFMAs issued after them (a scheduling barrier
+// keeps the compiler from pairing each read with its MFMA, which would expose
+// the LDS latency once per MFMA with a single wave on the SIMD).
 template <bool SPLIT, int KS, int NB>
 __device__ __forceinline__ void rows_mma(f32x16 (&total)[NB], const Frags<SPLIT, KS> &f,
                                          const unsigned short *rowHi, const unsigned short *rowLo,
                                          int blockStride) {
+    u32x4 bh[KS][NB];
+    u32x4 bl[SPLIT ? KS : 1][SPLIT ? NB : 1];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            bh[s][b] = *reinterpret_cast<const u32x4 *>(rowHi + b * blockStride + 16 * s);
+            if constexpr (SPLIT) bl[s][b] = *reinterpret_cast<const u32x4 *>(rowLo + b * blockStride + 16 * s);
+        }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
         const bf16x8 th = as_bf16x8(f.hi[s]);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-            const unsigned short *ph = rowHi + b * blockStride + 16 * s;
-            const bf16x8 fh = as_bf16x8(*reinterpret_cast<const u32x4 *>(ph));
+            const bf16x8 fh = as_bf16x8(bh[s][b]);
             if constexpr (SPLIT) {
-                const unsigned short *pl = rowLo + b * blockStride + 16 * s;
-                const bf16x8 fl = as_bf16x8(*reinterpret_cast<const u32x4 *>(pl));
+                const bf16x8 fl = as_bf16x8(bl[s][b]);
                 total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.lo[s]), fh, total[b], 0, 0, 0);
                 total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fl, total[b], 0, 0, 0);
                 total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.mid[s]), fh, total[b], 0, 0, 0);
@@ -254,7 +265,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             for (int b = 0; b < NB; ++b) total[b] = f32x16{};
             // kD fragment sets in flight: row dy + kD loads while row dy
             // multiplies (one row's MFMAs take less time than an L2 load)
-            constexpr int kD = SPLIT ? 3 : 4;
+            constexpr int kD = (SPLIT || KS > 6) ? 2 : 4;
             Frags<SPLIT, KS> f[kD];
 #pragma unroll
             for (int i = 0; i < kD; ++i)
