@@ -246,12 +246,23 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         const int yb = y0 + 32 * wave + n;
         // the owners' current state values, loaded now so that they arrive
         // while the window rows multiply
+        // (four consecutive columns per 16-byte load when W is a multiple of 4)
         float aCur[16];
         if (MODE == 0 && wave < NB) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
-                aCur[r] = (x < W && yb < H) ? src[(size_t)yb * W + x] : 0.0f;
+            for (int q = 0; q < 4; ++q) {
+                const int x = x0 + 8 * q + 4 * grp;
+                if (wd.vec4 && x < W && yb < H) {
+                    const float4 a4 = gld4(reinterpret_cast<const LENIA_GLOBAL float4 *>(src + (size_t)yb * W + x));
+                    aCur[4 * q] = a4.x;
+                    aCur[4 * q + 1] = a4.y;
+                    aCur[4 * q + 2] = a4.z;
+                    aCur[4 * q + 3] = a4.w;
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        aCur[4 * q + i] = (x + i < W && yb < H) ? src[(size_t)yb * W + x + i] : 0.0f;
+                }
             }
         }
         float gacc[16];
@@ -321,8 +332,19 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
-                if (x < W && yb < H)
-                    dst[(size_t)yb * W + x] = growth_finish(g, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
+                aCur[r] = growth_finish(g, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int x = x0 + 8 * q + 4 * grp;
+                if (wd.vec4 && x < W && yb < H) {
+                    gst4(reinterpret_cast<LENIA_GLOBAL float4 *>(dst + (size_t)yb * W + x),
+                         make_float4(aCur[4 * q], aCur[4 * q + 1], aCur[4 * q + 2], aCur[4 * q + 3]));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        if (x + i < W && yb < H) dst[(size_t)yb * W + x + i] = aCur[4 * q + i];
+                }
             }
         }
         __syncthreads();
