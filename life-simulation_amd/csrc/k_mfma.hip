@@ -54,7 +54,9 @@ constexpr int kMT = 64 * kMW;  // threads
 constexpr int kMaxSteps = 8;   // k-steps of 16 per window row (J <= 128, h <= 48)
 constexpr int kStage = 16;     // state-tile loads in flight per thread (one column each)
 constexpr int kStage4 = 8;     // the same, four columns each
-constexpr int kRedFloats = 2 * 64 * 16;  // reduction buffer: two waves' block totals
+// reduction buffer: the three non-owner partials of each output block
+template <bool SPLIT>
+constexpr int red_floats() { return 4 * 3 * 64 * 16 / (SPLIT ? 2 : 1); }
 
 // output blocks (32 rows each) per workgroup
 template <bool SPLIT>
@@ -293,28 +295,31 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
                 }
             }
             // fixed-order reduction of block b into wave b over the window-row
-            // quarters q = 0..3 (wave w holds quarter w): (q_b + q_b+2) + (q_b+1 + q_b+3)
-            float mine[16];
+            // quarters (wave b + i holds partial p_i of block b): every wave
+            // stores its partials of the blocks it does not own, one barrier,
+            // then owner b adds (p_0 + p_2) + (p_1 + p_3)
+            static_assert(blocks_y<SPLIT>() * 3 * 64 * 16 <= red_floats<SPLIT>(), "reduction buffer");
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int rel = (wave - b + kMW) & (kMW - 1);  // 0 = owner
-                if (rel >= 2)
+                if (rel != 0)
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) red[((rel - 2) * 16 + r) * 64 + lane] = total[b][r];
-                __syncthreads();
-                if (rel < 2)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) total[b][r] += red[(rel * 16 + r) * 64 + lane];
-                __syncthreads();
-                if (rel == 1)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) red[r * 64 + lane] = total[b][r];
-                __syncthreads();
-                if (rel == 0)
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) mine[r] = total[b][r] + red[r * 64 + lane];
-                __syncthreads();
+                    for (int r = 0; r < 16; ++r) red[((b * 3 + rel - 1) * 16 + r) * 64 + lane] = total[b][r];
             }
+            __syncthreads();
+            float mine[16];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                if (b == wave) {
+                    const float *p1 = red + (b * 3 + 0) * 16 * 64 + lane;
+                    const float *p2 = red + (b * 3 + 1) * 16 * 64 + lane;
+                    const float *p3 = red + (b * 3 + 2) * 16 * 64 + lane;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r)
+                        mine[r] = (total[b][r] + p2[r * 64]) + (p1[r * 64] + p3[r * 64]);
+                }
+            }
+            __syncthreads();  // the next kernel's partials reuse the buffer
             if (wave < NB) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
@@ -396,6 +401,10 @@ void mfma_tile_shape(bool split, int *tx, int *ty) {
 
 int mfma_max_ksteps() { return kMaxSteps; }
 
+// largest window half-width of each form (plain BF16 41, split-BF16 32): the
+// register budget of the fragment sets, and the tested range
+int mfma_max_half(bool split) { return split ? 32 : 41; }
+
 // columns between the staged tile's first column and x0 - h: the tile then
 // starts on a multiple of 4 (x0 is a multiple of 32), for 16-byte loads
 int mfma_shift(int h, int W) { return W % 4 == 0 ? (4 - h % 4) % 4 : 0; }
@@ -410,7 +419,7 @@ size_t mfma_lds_bytes(int h, int W, bool split) {
     const int pitch = 16 * ks + 8;
     const int rows = 32 * (split ? blocks_y<true>() : blocks_y<false>()) + 2 * h;
     return (size_t)rows * pitch * sizeof(unsigned short) * (split ? 2 : 1) +
-           kRedFloats * sizeof(float);
+           (size_t)(split ? red_floats<true>() : red_floats<false>()) * sizeof(float);
 }
 
 }  // namespace lenia

@@ -57,6 +57,7 @@ hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *wor
                        int kSel, float *out, hipStream_t s);
 void mfma_tile_shape(bool split, int *tx, int *ty);
 int mfma_max_ksteps();
+int mfma_max_half(bool split);
 int mfma_shift(int h, int W);
 int mfma_ksteps(int h, int W);
 size_t mfma_lds_bytes(int h, int W, bool split);

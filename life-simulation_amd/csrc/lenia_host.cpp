@@ -764,12 +764,13 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         for (uint32_t k = 0; k < desc->nKernels; ++k) h = std::max(h, kernel_half(desc->kernels[k]));
         const int ks = mfma_ksteps(h, W);
         const size_t lds = mfma_lds_bytes(h, W, w->split());
-        if (ks > mfma_max_ksteps() || lds > kLdsMax) {
+        if (h > mfma_max_half(w->split()) || ks > mfma_max_ksteps() || lds > kFftLdsMax) {
             delete w;
             return fail(LENIA_ERR_UNSUPPORTED,
-                        "MFMA form: window half-width %d needs %zu B of LDS per workgroup (limit %zu)%s",
-                        h, lds, (size_t)kLdsMax,
-                        w->split() ? "; the plain BF16 form or the FP32 forms serve it" : "");
+                        "MFMA form: window half-width %d exceeds the %s form's limit %d%s", h,
+                        w->split() ? "split-BF16" : "BF16", mfma_max_half(w->split()),
+                        w->split() ? "; the plain BF16 form or the FP32 forms serve it"
+                                   : "; the FP32 forms serve it");
         }
     }
     w->moduleSeed = 0;
