@@ -5,8 +5,8 @@
 // extended to up to four kernels with the leniaGrowthMulti rule
 // (Lenia.metal:161-232).
 //
-// A 128-thread workgroup owns a 64 (x) x 16 (y) output tile and stages its
-// (16+2h) x (64+P) toroidal halo in LDS once (P = window width rounded up to
+// A 256-thread workgroup owns a 64 (x) x 32 (y) output tile and stages its
+// (32+2h) x (64+P) toroidal halo in LDS once (P = window width rounded up to
 // 8; the window rows are zero padded to P).  Each thread produces 8
 // horizontally adjacent outputs of one row.  For every window row it slides
 // a 16-value register window along the staged row, 8 taps at a time: two
@@ -21,9 +21,9 @@
 
 namespace lenia {
 
-constexpr int kDT = 128;   // threads: 8 (x) x 16 (y)
+constexpr int kDT = 256;   // threads: 8 (x) x 32 (y)
 constexpr int kDTX = 64;   // output tile width
-constexpr int kDTY = 16;   // output tile height
+constexpr int kDTY = 32;   // output tile height
 constexpr int kDOX = 8;    // outputs per thread (x)
 constexpr int kDStage = 16;
 
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const StepClock ck = clock_at(wd.clock, stepOff);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
-        // halo rows y0-h .. y0+15+h, columns x0-h .. x0-h+cols-1 (toroidal;
+        // halo rows y0-h .. y0+31+h, columns x0-h .. x0-h+cols-1 (toroidal;
         // columns past x0+63+h only meet zero weights)
         const int nst = rows * cols;
         for (int base = 0; base < nst; base += kDT * kDStage) {

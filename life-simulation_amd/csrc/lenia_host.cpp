@@ -41,8 +41,6 @@ int lenia::hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
-// dynamic LDS of the direct forms' workgroups
-static constexpr size_t kLdsMax = 65536;
 
 extern "C" const char *lenia_last_error(void) { return g_err.c_str(); }
 extern "C" const char *lenia_version(void) { return "lenia-mi355x 0.1 (gfx950)"; }
@@ -335,11 +333,10 @@ static int check_desc(const lenia_world_desc *d) {
         if (kd.core == LENIA_CORE_GAUSSIAN && !(kd.shellSigma > 0.0f))
             return fail(LENIA_ERR_INVALID_ARG, "kernel %u: shellSigma must be > 0", k);
         int h = kernel_half(kd);
-        if (d->convMode == LENIA_CONV_DIRECT && direct_lds_bytes(h) > kLdsMax)
+        if (d->convMode == LENIA_CONV_DIRECT && (h > 43 || direct_lds_bytes(h) > kFftLdsMax))
             return fail(LENIA_ERR_UNSUPPORTED,
-                        "kernel %u: the FP32 direct form stages a (16+2h) x (68+P) LDS tile "
-                        "(P = 2h+1 rounded up to 8) and supports h <= 43 (got %d); use the FFT "
-                        "form", k, h);
+                        "kernel %u: the FP32 direct form supports window half-widths h <= 43 "
+                        "(got %d); use the FFT form", k, h);
         if (2 * h + 1 > (int)std::min(d->width, d->height))
             return fail(LENIA_ERR_INVALID_ARG,
                         "kernel %u: window 2*%d+1 exceeds the field (%ux%u); the torus would "
