@@ -17,17 +17,19 @@
 // precomputed on the host in register order (64 lanes x 8 bf16 per k-step)
 // and streamed from L2.
 //
-// The state tile (rows y0-h .. y0+32NB-1+h, 16 KS columns) is staged in LDS as
+// The state tile (rows y0-h .. y0+32NBY-1+h, 16 (KS+2) columns) is staged in LDS as
 // BF16, four columns per 16-byte load when W is a multiple of 4: the tile then
 // starts `shift` = (-h mod 4) columns left of x0-h, so every load is aligned
 // and none straddles the toroidal wrap, and the fragments carry the shift
 // (K[dy][j - m - shift]).
 //
-// Work split: a workgroup owns a 32 (x) x 32 NB (y) output tile (NB blocks
-// stacked in y).  Its 4 waves split the 2h+1 window rows into quarters; each
-// wave applies every fragment it loads to all NB blocks (NB independent
-// accumulation chains), the quarters' sums are reduced through LDS in a fixed
-// order, and wave b finishes block b.
+// Work split: a workgroup owns a 64 (x) x 32 NBY (y) output tile, NB = 2 NBY
+// blocks of 32 x 32 (NBY = 2 plain, 1 split).  Its 4 waves split the 2h+1
+// window rows into quarters; each wave applies every fragment it loads to the
+// blocks of both columns (the right column at the fragment of k-step s - 2) and
+// every state operand it reads to both columns, NB independent accumulation
+// chains; the quarters' sums are reduced through LDS in a fixed order, and
+// wave b finishes block b.
 //
 // Split-BF16 (BF16X3): the state in two terms, F = Fhi + Flo (hi = bf16(v),
 // lo = bf16(v - hi), residual <= 2^-18 |v|), the kernel in three, T = Thi +
@@ -54,13 +56,21 @@ constexpr int kMT = 64 * kMW;  // threads
 constexpr int kMaxSteps = 8;   // k-steps of 16 per window row (J <= 128, h <= 48)
 constexpr int kStage = 16;     // state-tile loads in flight per thread (one column each)
 constexpr int kStage4 = 8;     // the same, four columns each
+
+// output blocks of 32 x 32 per workgroup: NBX side by side in x, NBY stacked
+// in y.  The NBX blocks of a row share every state operand: block bx at k-step
+// s uses the kernel fragment of k-step s - 2 bx (its columns are 32 = 2 x 16
+// further right), so a state operand read from LDS feeds NBX MFMAs.
+template <bool SPLIT>
+constexpr int blocks_x() { return 2; }
+template <bool SPLIT>
+constexpr int blocks_y() { return SPLIT ? 1 : 2; }
+template <bool SPLIT>
+constexpr int blocks_all() { return blocks_x<SPLIT>() * blocks_y<SPLIT>(); }
+
 // reduction buffer: the three non-owner partials of each output block
 template <bool SPLIT>
-constexpr int red_floats() { return 4 * 3 * 64 * 16 / (SPLIT ? 2 : 1); }
-
-// output blocks (32 rows each) per workgroup
-template <bool SPLIT>
-constexpr int blocks_y() { return SPLIT ? 2 : 4; }
+constexpr int red_floats() { return blocks_all<SPLIT>() * 3 * 64 * 16; }
 
 // round-to-nearest-even FP32 -> BF16 bits (finite inputs)
 __device__ __forceinline__ unsigned short bf16_bits(float v) {
@@ -100,38 +110,46 @@ __device__ __forceinline__ void load_toeplitz(Frags<SPLIT, KS> &f, const MfmaWor
     }
 }
 
-// one window row applied to the NB blocks, accumulated into total[b];
-// rowHi/rowLo point at the row of block 0.  All state operands of the row are
-// read from LDS first and the MFMAs issued after them (a scheduling barrier
-// keeps the compiler from pairing each read with its MFMA, which would expose
-// the LDS latency once per MFMA with a single wave on the SIMD).
-template <bool SPLIT, int KS, int NB>
-__device__ __forceinline__ void rows_mma(f32x16 (&total)[NB], const Frags<SPLIT, KS> &f,
+// one window row applied to the NBX x NBY blocks, accumulated into
+// total[bx NBY + by]; rowHi/rowLo point at the row of block row 0.  All state
+// operands of the row are read from LDS first and the MFMAs issued after them
+// (a scheduling barrier keeps the compiler from pairing each read with its
+// MFMA, which would expose the LDS latency once per MFMA with a single wave on
+// the SIMD).
+template <bool SPLIT, int KS, int NBX, int NBY>
+__device__ __forceinline__ void rows_mma(f32x16 (&total)[NBX * NBY], const Frags<SPLIT, KS> &f,
                                          const unsigned short *rowHi, const unsigned short *rowLo,
                                          int blockStride) {
-    u32x4 bh[KS][NB];
-    u32x4 bl[SPLIT ? KS : 1][SPLIT ? NB : 1];
+    constexpr int KSX = KS + 2 * (NBX - 1);  // state k-steps a row of blocks spans
+    u32x4 bh[KSX][NBY];
+    u32x4 bl[SPLIT ? KSX : 1][SPLIT ? NBY : 1];
 #pragma unroll
-    for (int s = 0; s < KS; ++s)
+    for (int s = 0; s < KSX; ++s)
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            bh[s][b] = *reinterpret_cast<const u32x4 *>(rowHi + b * blockStride + 16 * s);
-            if constexpr (SPLIT) bl[s][b] = *reinterpret_cast<const u32x4 *>(rowLo + b * blockStride + 16 * s);
+        for (int by = 0; by < NBY; ++by) {
+            bh[s][by] = *reinterpret_cast<const u32x4 *>(rowHi + by * blockStride + 16 * s);
+            if constexpr (SPLIT) bl[s][by] = *reinterpret_cast<const u32x4 *>(rowLo + by * blockStride + 16 * s);
         }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-        const bf16x8 th = as_bf16x8(f.hi[s]);
+    for (int s = 0; s < KSX; ++s) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const bf16x8 fh = as_bf16x8(bh[s][b]);
-            if constexpr (SPLIT) {
-                const bf16x8 fl = as_bf16x8(bl[s][b]);
-                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.lo[s]), fh, total[b], 0, 0, 0);
-                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fl, total[b], 0, 0, 0);
-                total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.mid[s]), fh, total[b], 0, 0, 0);
+        for (int bx = 0; bx < NBX; ++bx) {
+            const int t = s - 2 * bx;  // kernel k-step of this block
+            if (t < 0 || t >= KS) continue;
+            const bf16x8 th = as_bf16x8(f.hi[t]);
+#pragma unroll
+            for (int by = 0; by < NBY; ++by) {
+                f32x16 &acc = total[bx * NBY + by];
+                const bf16x8 fh = as_bf16x8(bh[s][by]);
+                if constexpr (SPLIT) {
+                    const bf16x8 fl = as_bf16x8(bl[s][by]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.lo[t]), fh, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fl, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(f.mid[t]), fh, acc, 0, 0, 0);
+                }
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fh, acc, 0, 0, 0);
             }
-            total[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(th, fh, total[b], 0, 0, 0);
         }
     }
 }
@@ -143,7 +161,8 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
                                               const int2 *__restrict__ tiles, int nTiles,
                                               uint32_t stepOff, int kSel,
                                               float *__restrict__ out) {
-    constexpr int NB = blocks_y<SPLIT>();
+    constexpr int NBX = blocks_x<SPLIT>(), NBY = blocks_y<SPLIT>(), NB = NBX * NBY;
+    constexpr int KSX = KS + 2 * (NBX - 1);
     extern __shared__ unsigned short smemMf[];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -151,9 +170,9 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         const int2 tile = tiles[ti];
         const MfmaWorldDev &wd = worlds[tile.x];
         const int W = wd.W, H = wd.H, h = wd.half;
-        const int x0 = (tile.y % wd.tilesX) * 32;
-        const int y0 = (tile.y / wd.tilesX) * (32 * NB);
-        const int rows = 32 * NB + 2 * h, cols = 16 * KS, pitch = wd.pitch;
+        const int x0 = (tile.y % wd.tilesX) * (32 * NBX);
+        const int y0 = (tile.y / wd.tilesX) * (32 * NBY);
+        const int rows = 32 * NBY + 2 * h, cols = 16 * KSX, pitch = wd.pitch;
         const int plane = rows * pitch;
         unsigned short *tHi = smemMf;
         unsigned short *tLo = smemMf + plane;
@@ -169,7 +188,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             // W and xs multiples of 4: aligned 16-byte loads that never
             // straddle the wrap; rows and column groups wrap by one period
             // except near a field smaller than the tile (then by modulo)
-            constexpr int G4 = 4 * KS;  // float4 groups per staged row
+            constexpr int G4 = 4 * KSX;  // float4 groups per staged row
             const int ngr = rows * G4;
             for (int base = 0; base < ngr; base += kMT * kStage4) {
                 float4 v[kStage4];
@@ -243,9 +262,11 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         const int r0 = (nrows * wave) / kMW, r1 = (nrows * (wave + 1)) / kMW;
         const unsigned short *colHi = tHi + n * pitch + 8 * grp;
         const unsigned short *colLo = tLo + n * pitch + 8 * grp;
-        // wave b < NB finishes output block b: its rows y0 + 32 b + n, columns
-        // x0 + 8 (r/4) + 4 grp + (r%4) of lane slot r
-        const int yb = y0 + 32 * wave + n;
+        // wave b < NB finishes output block b = (b / NBY, b % NBY): its rows
+        // y0 + 32 (b % NBY) + n, columns xq + 8 (r/4) + 4 grp + (r%4) of lane
+        // slot r with xq = x0 + 32 (b / NBY)
+        const int yb = y0 + 32 * (wave % NBY) + n;
+        const int xq = x0 + 32 * (wave / NBY);
         // the owners' current state values, loaded now so that they arrive
         // while the window rows multiply
         // (four consecutive columns per 16-byte load when W is a multiple of 4)
@@ -253,7 +274,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         if (MODE == 0 && wave < NB) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int x = x0 + 8 * q + 4 * grp;
+                const int x = xq + 8 * q + 4 * grp;
                 if (wd.vec4 && x < W && yb < H) {
                     const float4 a4 = gld4(reinterpret_cast<const LENIA_GLOBAL float4 *>(src + (size_t)yb * W + x));
                     aCur[4 * q] = a4.x;
@@ -288,7 +309,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
                 for (int i = 0; i < kD; ++i) {
                     const int row = dy + i;
                     if (row < r1) {
-                        rows_mma<SPLIT, KS, NB>(total, f[i], colHi + row * pitch, colLo + row * pitch,
+                        rows_mma<SPLIT, KS, NBX, NBY>(total, f[i], colHi + row * pitch, colLo + row * pitch,
                                                 32 * pitch);
                         if (row + kD < r1) load_toeplitz<SPLIT, KS>(f[i], wd, k, row + kD, lane);
                     }
@@ -298,7 +319,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             // quarters (wave b + i holds partial p_i of block b): every wave
             // stores its partials of the blocks it does not own, one barrier,
             // then owner b adds (p_0 + p_2) + (p_1 + p_3)
-            static_assert(blocks_y<SPLIT>() * 3 * 64 * 16 <= red_floats<SPLIT>(), "reduction buffer");
+            static_assert(NB * 3 * 64 * 16 <= red_floats<SPLIT>(), "reduction buffer");
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int rel = (wave - b + kMW) & (kMW - 1);  // 0 = owner
@@ -323,7 +344,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             if (wave < NB) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
+                    const int x = xq + 8 * (r >> 2) + 4 * grp + (r & 3);
                     if (MODE == 1) {
                         if (x < W && yb < H) gp(out)[(size_t)yb * W + x] = mine[r];
                     } else if (x < W && yb < H) {
@@ -336,12 +357,12 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
         if (MODE == 0 && wave < NB) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int x = x0 + 8 * (r >> 2) + 4 * grp + (r & 3);
+                const int x = xq + 8 * (r >> 2) + 4 * grp + (r & 3);
                 aCur[r] = growth_finish(g, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int x = x0 + 8 * q + 4 * grp;
+                const int x = xq + 8 * q + 4 * grp;
                 if (wd.vec4 && x < W && yb < H) {
                     gst4(reinterpret_cast<LENIA_GLOBAL float4 *>(dst + (size_t)yb * W + x),
                          make_float4(aCur[4 * q], aCur[4 * q + 1], aCur[4 * q + 2], aCur[4 * q + 3]));
@@ -395,7 +416,7 @@ hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *wor
 }
 
 void mfma_tile_shape(bool split, int *tx, int *ty) {
-    *tx = 32;
+    *tx = 32 * (split ? blocks_x<true>() : blocks_x<false>());
     *ty = 32 * (split ? blocks_y<true>() : blocks_y<false>());
 }
 
@@ -414,9 +435,14 @@ int mfma_shift(int h, int W) { return W % 4 == 0 ? (4 - h % 4) % 4 : 0; }
 // shift <= 1 when 2h = 14 mod 16)
 int mfma_ksteps(int h, int W) { return (32 + 2 * h + mfma_shift(h, W) + 15) / 16; }
 
+// staged state columns of a tile: the kernel's k-steps plus two per extra
+// block in x (the forms share blocks_x)
+int mfma_tile_cols(int ksteps) { return 16 * (ksteps + 2 * (blocks_x<false>() - 1)); }
+static_assert(blocks_x<false>() == blocks_x<true>(), "one column count for both forms");
+
 size_t mfma_lds_bytes(int h, int W, bool split) {
     const int ks = mfma_ksteps(h, W);
-    const int pitch = 16 * ks + 8;
+    const int pitch = mfma_tile_cols(ks) + 8;
     const int rows = 32 * (split ? blocks_y<true>() : blocks_y<false>()) + 2 * h;
     return (size_t)rows * pitch * sizeof(unsigned short) * (split ? 2 : 1) +
            (size_t)(split ? red_floats<true>() : red_floats<false>()) * sizeof(float);

@@ -60,6 +60,7 @@ int mfma_max_ksteps();
 int mfma_max_half(bool split);
 int mfma_shift(int h, int W);
 int mfma_ksteps(int h, int W);
+int mfma_tile_cols(int ksteps);
 size_t mfma_lds_bytes(int h, int W, bool split);
 
 // adjoint (k_adjoint.hip)

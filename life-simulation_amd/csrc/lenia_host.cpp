@@ -662,7 +662,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         md.tilesX = (md.W + tx - 1) / tx;
         md.tilesY = (md.H + ty - 1) / ty;
         md.ksteps = w->ksteps;
-        md.tileCols = 16 * w->ksteps;
+        md.tileCols = mfma_tile_cols(w->ksteps);
         md.pitch = md.tileCols + 8;
         md.vec4 = md.W % 4 == 0;
         md.shift = mfma_shift(w->half, md.W);

@@ -143,7 +143,7 @@ struct MfmaWorldDev {
     int32_t half;       // window half-width (max over kernels)
     int32_t tilesX, tilesY;
     int32_t ksteps;     // 16-wide k-steps per window row: ceil((32 + 2 half) / 16)
-    int32_t tileCols;   // staged tile columns: 16 ksteps
+    int32_t tileCols;   // staged tile columns: 16 (ksteps + 2), two blocks side by side
     int32_t pitch;      // LDS row pitch in bf16 (tileCols + 8)
     int32_t vec4;       // W % 4 == 0: the state tile is staged four columns per load
     int32_t shift;      // staged columns start at x0 - half - shift (a multiple of 4 when vec4)
