@@ -67,3 +67,21 @@ def test_batch_autotune_keeps_bits(ctx):
     finally:
         for w in worlds + alone:
             w.close()
+
+
+def test_row_pass_workgroup_sizes_same_bits(ctx):
+    """the headline world stepped with each row-pass workgroup size the
+    library offers (1024 threads: side-by-side kernel transforms only) gives
+    the same bytes"""
+    import lenia
+
+    out = {}
+    for rt in (128, 256, 512, 1024):
+        d = dict(config1(), rowThreads=rt, rowPairsPerTile=1)
+        with lenia.World(ctx, d) as w:
+            assert w.resolved()["rowThreads"] == rt
+            w.reset(7777)
+            w.frame(0, 1)
+            w.step(12)
+            out[rt] = w.read().tobytes()
+    assert len(set(out.values())) == 1
