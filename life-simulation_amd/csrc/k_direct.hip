@@ -10,7 +10,9 @@
 // 8; the window rows are zero padded to P).  Each thread produces 8
 // horizontally adjacent outputs of one row.  For every window row it slides
 // a 16-value register window along the staged row, 8 taps at a time: two
-// 16-byte LDS reads and 8 weights feed 64 FMAs, so the loop is VALU-bound.
+// 16-byte LDS reads of state and two of weights (the kernel's window is
+// staged in LDS too) feed 64 FMAs, with the next chunk's reads issued before
+// the current chunk's FMAs.
 // Each output's sum is taken per window row first, then across rows
 // (two-level summation keeps the FP32 potential within ~1e-6 of the FP64
 // value for 3000-tap windows).
@@ -45,21 +47,9 @@ __device__ __forceinline__ void lds_read8(const float4 *p, float (&v)[8]) {
     v[7] = b.w;
 }
 
-// window weights are the same for every lane: read them through the
-// constant address space, so they arrive as scalar loads into SGPRs
+// per-row chunk ranges are the same for every lane: read them through the
+// constant address space, so they arrive as scalar loads
 #define LENIA_CONST __attribute__((address_space(4)))
-typedef float cv4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void const_read8(const LENIA_CONST cv4 *p, float (&w)[8]) {
-    const cv4 a = p[0], b = p[1];
-    w[0] = a.x;
-    w[1] = a.y;
-    w[2] = a.z;
-    w[3] = a.w;
-    w[4] = b.x;
-    w[5] = b.y;
-    w[6] = b.z;
-    w[7] = b.w;
-}
 
 // 8 taps of one window row: out[i] += w[t] * in[i + t], in = lo ++ hi
 __device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const float (&lo)[8],
@@ -122,36 +112,57 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const int kEnd = MODE == 1 ? kSel + 1 : (int)g.nKernels;
         const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
         const int pitch4 = pitch / 4;
+        float *winS = tileS + rows * pitch;  // the current kernel's window, [n][np]
         for (int k = kBeg; k < kEnd; ++k) {
-            const LENIA_CONST cv4 *wk = (const LENIA_CONST cv4 *)(wd.win) + (size_t)k * n * (np / 4);
+            // stage kernel k's window in LDS: every load of the tap loop is
+            // then an LDS load, and those complete in order, so the loop can
+            // wait for the oldest of its loads while the newer ones fly
+            __syncthreads();  // the previous kernel's window is no longer read
+            {
+                const LENIA_GLOBAL float4 *wg =
+                    gp(reinterpret_cast<const float4 *>(wd.win)) + (size_t)k * n * (np / 4);
+                float4 *ws4 = reinterpret_cast<float4 *>(winS);
+                for (int i = tid; i < n * (np / 4); i += kDT) ws4[i] = gld4(wg + i);
+            }
+            __syncthreads();
             float tot[kDOX];
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
             const LENIA_CONST int *rc = (const LENIA_CONST int *)(wd.rowChunks) + (size_t)2 * k * n;
             for (int iy = 0; iy < n; ++iy) {
                 const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
-                const LENIA_CONST cv4 *wrow = wk + (size_t)iy * (np / 4);
+                const float4 *wrow = reinterpret_cast<const float4 *>(winS) + iy * (np / 4);
                 // chunks [c0, c1) of this row hold its non-zero weights
                 const int c1 = rc[2 * iy + 1];
                 float rs[kDOX];
 #pragma unroll
                 for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
-                float va[8], vb[8], w[8];
+                // va, vb: state chunks c, c+1; w0: weights of chunk c.  Each
+                // iteration loads chunk c+2 and its weights before the taps of
+                // chunk c, and chunk c+3 and the weights of c+2 before the
+                // taps of chunk c+1 (reads stay inside the row: chunk c+3 is
+                // loaded only when c+3 <= c1)
+                float va[8], vb[8], w0[8];
                 int c = rc[2 * iy];
                 lds_read8(trow + 2 * c, va);
+                lds_read8(trow + 2 * c + 2, vb);
+                lds_read8(wrow + 2 * c, w0);
                 for (; c + 1 < c1; c += 2) {
-                    lds_read8(trow + 2 * c + 2, vb);
-                    const_read8(wrow + 2 * c, w);
-                    taps8(rs, w, va, vb);
-                    lds_read8(trow + 2 * c + 4, va);
-                    const_read8(wrow + 2 * c + 2, w);
-                    taps8(rs, w, vb, va);
+                    float vc[8], w1[8];
+                    lds_read8(trow + 2 * c + 4, vc);
+                    lds_read8(wrow + 2 * c + 2, w1);
+                    taps8(rs, w0, va, vb);
+                    if (c + 3 <= c1) lds_read8(trow + 2 * c + 6, va);
+                    if (c + 2 < c1) lds_read8(wrow + 2 * c + 4, w0);
+                    taps8(rs, w1, vb, vc);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const float t = va[i];
+                        va[i] = vc[i];
+                        vb[i] = t;
+                    }
                 }
-                if (c < c1) {
-                    lds_read8(trow + 2 * c + 2, vb);
-                    const_read8(wrow + 2 * c, w);
-                    taps8(rs, w, va, vb);
-                }
+                if (c < c1) taps8(rs, w0, va, vb);
 #pragma unroll
                 for (int i = 0; i < kDOX; ++i) tot[i] += rs[i];
             }
@@ -202,7 +213,8 @@ int direct_npad(int h) { return (2 * h + 1 + 7) & ~7; }
 
 size_t direct_lds_bytes(int h) {
     const int pitch = kDTX + direct_npad(h) + 4;
-    return (size_t)(kDTY + 2 * h) * pitch * sizeof(float);
+    // the halo tile, then one kernel window [2h+1][npad]
+    return ((size_t)(kDTY + 2 * h) * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float);
 }
 
 }  // namespace lenia
