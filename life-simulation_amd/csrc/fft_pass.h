@@ -396,7 +396,8 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                 int e = tid + i * T;
                 if (e < nk) {
                     int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
-                    sr[i] = S0[(size_t)k * sStride + c];
+                    // (k, c) offsets fit 32 bits (K Wh H < 2^31): no 64-bit multiply
+                    sr[i] = S0[k * (int)sStride + c];
                 }
             }
             __syncthreads();
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             fft_run<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
             for (int e = tid; e < nk; e += T) {
                 int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
-                gst2(G0 + (size_t)k * sStride + c, Y[sw(e)]);
+                gst2(G0 + (k * (int)sStride + c), Y[sw(e)]);
             }
             __syncthreads();
             continue;
