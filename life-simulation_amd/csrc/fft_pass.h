@@ -73,47 +73,48 @@ struct TwCopy {
 // gather_issue puts the loads in flight; gather_store writes LDS.
 constexpr int kMG = kMaxV / 2 + 1;  // float4 loads per thread
 
+// Slot geometry kept between the two: zi = LDS index of (slot s, row pair
+// p, column kx), kx (-1: no load in this slot).
 template <int T>
-__device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], const LENIA_GLOBAL float4 *G4,
-                                             size_t gStride, int kBeg, int kn, int H2, int nt,
-                                             int P, uint32_t mP, int nm, int tid) {
+__device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], int (&zi)[kMG], int (&kxv)[kMG],
+                                             const LENIA_GLOBAL float4 *G4, size_t gStride,
+                                             int kBeg, int kn, int H2, int nt, int P, uint32_t mP,
+                                             int nm, int W, int tid) {
     const int total = kn * nm;
 #pragma unroll
     for (int i = 0; i < kMG; ++i) {
         int m = tid + i * T;
-        if (m < total) {
-            int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
-            int mm = m - s * nm;
-            int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
-            if (p < nt) gv[i] = gld4(G4 + (size_t)(kBeg + s) * gStride + kx * H2 + p);
-        }
-    }
-}
-
-template <int T>
-__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[kMG], int kn, int W,
-                                             int nt, int P, uint32_t mP, int nm, int tid) {
-    const int total = kn * nm;
-#pragma unroll
-    for (int i = 0; i < kMG; ++i) {
-        int m = tid + i * T;
+        kxv[i] = -1;
         if (m < total) {
             int s = (m >= nm) + (m >= 2 * nm) + (m >= 3 * nm);
             int mm = m - s * nm;
             int kx = (int)fast_div((uint32_t)mm, mP), p = mm - kx * P;
             if (p < nt) {
-                float4 g4 = gv[i];
-                if (kx == 0 || kx == W / 2) {
-                    // real columns for a real field: drop the rounding
-                    // residue instead of leaking it across rows
-                    g4.y = 0.0f;
-                    g4.w = 0.0f;
-                }
-                const int base = (s * nt + p) * W;
-                Z[sw(base + kx)] = make_float2(g4.x - g4.w, g4.y + g4.z);
-                if (kx > 0 && kx < W / 2)
-                    Z[sw(base + W - kx)] = make_float2(g4.x + g4.w, g4.z - g4.y);
+                gv[i] = gld4(G4 + (size_t)(kBeg + s) * gStride + kx * H2 + p);
+                kxv[i] = kx;
+                zi[i] = (s * nt + p) * W + kx;
             }
+        }
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[kMG], const int (&zi)[kMG],
+                                             const int (&kxv)[kMG], int W) {
+#pragma unroll
+    for (int i = 0; i < kMG; ++i) {
+        const int kx = kxv[i];
+        if (kx >= 0) {
+            float4 g4 = gv[i];
+            if (kx == 0 || kx == W / 2) {
+                // real columns for a real field: drop the rounding
+                // residue instead of leaking it across rows
+                g4.y = 0.0f;
+                g4.w = 0.0f;
+            }
+            Z[sw(zi[i])] = make_float2(g4.x - g4.w, g4.y + g4.z);
+            if (kx > 0 && kx < W / 2)
+                Z[sw(zi[i] + W - 2 * kx)] = make_float2(g4.x + g4.w, g4.z - g4.y);
         }
     }
 }
@@ -206,9 +207,10 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
             const int kBeg = MODE == 2 ? kSel : 0;
             const int kn = MODE == 2 ? 1 : (int)g.nKernels;
             float4 gv[kMG];
-            gather_issue<T>(gv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, tid);
+            int zi[kMG], kxv[kMG];
+            gather_issue<T>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
             twc.store(tw, twg, W, tid);
-            gather_store<T>(Z, gv, kn, W, nt, P, mP, nm, tid);
+            gather_store<T>(Z, gv, zi, kxv, W);
             __syncthreads();
             fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
             if (MODE == 2) {
@@ -282,13 +284,14 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
 #pragma unroll
             for (int i = 0; i < kMaxV; ++i) sa[i] = sb[i] = 0.0f;
             float4 gv[kMG];
-            gather_issue<T>(gv, G4, gStride, 0, 1, H2, nt, P, mP, nm, tid);
+            int zi[kMG], kxv[kMG];
+            gather_issue<T>(gv, zi, kxv, G4, gStride, 0, 1, H2, nt, P, mP, nm, W, tid);
             twc.store(tw, twg, W, tid);
             for (int k = 0; k < (int)g.nKernels; ++k) {
-                gather_store<T>(Z, gv, 1, W, nt, P, mP, nm, tid);
+                gather_store<T>(Z, gv, zi, kxv, W);
                 // the next kernel's spectrum loads fly during this transform
                 if (k + 1 < (int)g.nKernels)
-                    gather_issue<T>(gv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, tid);
+                    gather_issue<T>(gv, zi, kxv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, W, tid);
                 __syncthreads();
                 // table-driven transform inside the runtime kernel loop: the
                 // compile-time one would keep its addresses live across it
