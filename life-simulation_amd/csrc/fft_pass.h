@@ -38,6 +38,12 @@ namespace lenia {
 
 constexpr int kMaxV = 16;  // complex values per thread per stage
 
+// Cells per thread held in registers by the row pass (state, growth sums):
+// 1024-thread workgroups keep 4, so the values live across the transforms
+// fit their 128-register budget without spilling; the host's shape check
+// admits such a launch only when (K) nt W <= 4 T.
+constexpr int row_values(int T) { return T >= 1024 ? 4 : kMaxV; }
+
 __device__ __forceinline__ int tile_of(int ti, int nTiles) {
     return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
 }
@@ -71,18 +77,17 @@ struct TwCopy {
 // full-length complex rows Z = Ga + i Gb, one LDS slot of nt rows per kernel,
 // with the Hermitian symmetry of each real row's spectrum for kx > W/2.
 // gather_issue puts the loads in flight; gather_store writes LDS.
-constexpr int kMG = kMaxV / 2 + 1;  // float4 loads per thread
-
+// NG = row_values(T) / 2 + 1 float4 loads per thread.
 // Slot geometry kept between the two: zi = LDS index of (slot s, row pair
 // p, column kx), kx (-1: no load in this slot).
-template <int T>
-__device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], int (&zi)[kMG], int (&kxv)[kMG],
+template <int T, int NG>
+__device__ __forceinline__ void gather_issue(float4 (&gv)[NG], int (&zi)[NG], int (&kxv)[NG],
                                              const LENIA_GLOBAL float4 *G4, size_t gStride,
                                              int kBeg, int kn, int H2, int nt, int P, uint32_t mP,
                                              int nm, int W, int tid) {
     const int total = kn * nm;
 #pragma unroll
-    for (int i = 0; i < kMG; ++i) {
+    for (int i = 0; i < NG; ++i) {
         int m = tid + i * T;
         kxv[i] = -1;
         if (m < total) {
@@ -98,11 +103,11 @@ __device__ __forceinline__ void gather_issue(float4 (&gv)[kMG], int (&zi)[kMG], 
     }
 }
 
-template <int T>
-__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[kMG], const int (&zi)[kMG],
-                                             const int (&kxv)[kMG], int W) {
+template <int NG>
+__device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[NG], const int (&zi)[NG],
+                                             const int (&kxv)[NG], int W) {
 #pragma unroll
-    for (int i = 0; i < kMG; ++i) {
+    for (int i = 0; i < NG; ++i) {
         const int kx = kxv[i];
         if (kx >= 0) {
             float4 g4 = gv[i];
@@ -152,6 +157,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                                                 uint32_t stepOff, int kSel,
                                                 float *__restrict__ out) {
     extern __shared__ float2 smem[];
+    constexpr int kV = row_values(T), kG = kV / 2 + 1;
     const int tid = threadIdx.x;
     const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
     if (ti < nTiles) do {
@@ -190,10 +196,10 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
         const uint32_t seed = clock_seed(ck, g.moduleSeed);
         const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
         const size_t gStride = (size_t)Wh * H2;
-        float ar0[kMaxV], ar1[kMaxV];
+        float ar0[kV], ar1[kV];
         if (MODE == 1) {
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < kV; ++i) {
                 int q = tid + i * T;
                 if (q < nq) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -206,11 +212,11 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
         if (KPAR || MODE == 2) {
             const int kBeg = MODE == 2 ? kSel : 0;
             const int kn = MODE == 2 ? 1 : (int)g.nKernels;
-            float4 gv[kMG];
-            int zi[kMG], kxv[kMG];
-            gather_issue<T>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
+            float4 gv[kG];
+            int zi[kG], kxv[kG];
+            gather_issue<T, kG>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
             twc.store(tw, twg, W, tid);
-            gather_store<T>(Z, gv, zi, kxv, W);
+            gather_store<kG>(Z, gv, zi, kxv, W);
             __syncthreads();
             fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
             if (MODE == 2) {
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                 continue;
             }
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < kV; ++i) {
                 int q = tid + i * T;
                 if (q < nq) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
             }
             // stores after all LDS reads (no read waits behind a store)
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < kV; ++i) {
                 int q = tid + i * T;
                 if (q < nq) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -280,24 +286,24 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                 }
             }
         } else {
-            float sa[kMaxV], sb[kMaxV];
+            float sa[kV], sb[kV];
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) sa[i] = sb[i] = 0.0f;
-            float4 gv[kMG];
-            int zi[kMG], kxv[kMG];
-            gather_issue<T>(gv, zi, kxv, G4, gStride, 0, 1, H2, nt, P, mP, nm, W, tid);
+            for (int i = 0; i < kV; ++i) sa[i] = sb[i] = 0.0f;
+            float4 gv[kG];
+            int zi[kG], kxv[kG];
+            gather_issue<T, kG>(gv, zi, kxv, G4, gStride, 0, 1, H2, nt, P, mP, nm, W, tid);
             twc.store(tw, twg, W, tid);
             for (int k = 0; k < (int)g.nKernels; ++k) {
-                gather_store<T>(Z, gv, zi, kxv, W);
+                gather_store<kG>(Z, gv, zi, kxv, W);
                 // the next kernel's spectrum loads fly during this transform
                 if (k + 1 < (int)g.nKernels)
-                    gather_issue<T>(gv, zi, kxv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, W, tid);
+                    gather_issue<T, kG>(gv, zi, kxv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, W, tid);
                 __syncthreads();
                 // table-driven transform inside the runtime kernel loop: the
                 // compile-time one would keep its addresses live across it
                 fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
 #pragma unroll
-                for (int i = 0; i < kMaxV; ++i) {
+                for (int i = 0; i < kV; ++i) {
                     int q = tid + i * T;
                     if (q < nq) {
                         int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                 __syncthreads();
             }
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < kV; ++i) {
                 int q = tid + i * T;
                 if (q < nq) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;

@@ -128,5 +128,6 @@ hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double
 }
 
 int fft_max_values() { return kMaxV; }
+int fft_row_values(int T) { return row_values(T); }
 
 }  // namespace lenia

@@ -42,6 +42,8 @@ hipError_t launch_stage_twiddles(const FftPlan &plan, const float *raw, float *o
 hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double *Cw, float *S,
                                   hipStream_t s);
 int fft_max_values();
+// cells per thread the row pass holds in registers at T threads
+int fft_row_values(int T);
 
 // direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,

@@ -543,6 +543,7 @@ bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpa
     const int mv = fft_max_values();
     const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
     if (lds > kFftLdsMax) return false;
+    if (rows && (long)(kpar ? nt * K : nt) * p.n > (long)T * fft_row_values(T)) return false;
     return plan_fits(p, kpar ? nt * K : nt, T, mv) && plan_fits(p, nt, T, mv);
 }
 
