@@ -85,3 +85,18 @@ def test_row_pass_workgroup_sizes_same_bits(ctx):
             w.step(12)
             out[rt] = w.read().tobytes()
     assert len(set(out.values())) == 1
+
+
+@pytest.mark.gpu
+def test_row_pass_1024_register_cap(ctx):
+    """1024-thread row passes hold at most 4 cells per thread: a tile of two
+    row pairs of the 1024-wide K=3 world (3 x 2 x 1024 > 4 x 1024) is refused
+    with an error, not run with cells left out"""
+    import lenia
+
+    d = dict(config1(), rowThreads=1024, rowPairsPerTile=2)
+    with pytest.raises(lenia.LeniaError, match="row-pass shape"):
+        lenia.World(ctx, d)
+    d = dict(config1(), rowThreads=512, rowPairsPerTile=2)
+    with lenia.World(ctx, d) as w:
+        assert w.resolved()["rowThreads"] == 512
