@@ -252,10 +252,8 @@ def measured_traffic(kernel: str):
         if ":" in line and line.strip().startswith("- "):
             k, v = line.strip()[2:].split(":", 1)
             vals[k.strip()] = v.strip()
-    if vals.get("kernel") != kernel:
-        return None
     try:
-        return int(vals["hbm_bytes_per_launch"])
+        return int(vals[kernel + " hbm_bytes_per_launch"])
     except (KeyError, ValueError):
         return None
 

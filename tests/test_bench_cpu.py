@@ -1,5 +1,6 @@
 """bench.py's command-line contract that needs no GPU: more than one GPU is
-refused with a message and exit status 2, and the option list is stable."""
+refused with a message and exit status 2, the option list is stable, and the
+roofline finds measured traffic for both FFT passes."""
 import os
 import subprocess
 import sys
@@ -25,3 +26,15 @@ def test_options():
     for opt in ("--gpus", "--steps", "--warmup", "--tiles", "--only", "--sweep-steps",
                 "--plain-steps", "--copy-calibration", "--no-cpu"):
         assert opt in r.stdout
+
+
+def test_traffic_for_either_dominant_pass():
+    """the roofline's traffic comes from the committed counter summary for
+    whichever pass measures longer (the two are within a few percent)"""
+    sys.path.insert(0, ROOT)
+    import bench
+
+    for k in ("k_fft_rows", "k_fft_cols"):
+        t = bench.measured_traffic(k)
+        assert isinstance(t, int) and 10_000_000 < t < 20_000_000
+    assert bench.measured_traffic("k_unknown") is None
