@@ -39,10 +39,13 @@ namespace lenia {
 constexpr int kMaxV = 16;  // complex values per thread per stage
 
 // Cells per thread held in registers by the row pass (state, growth sums):
-// 1024-thread workgroups keep 4, so the values live across the transforms
-// fit their 128-register budget without spilling; the host's shape check
-// admits such a launch only when (K) nt W <= 4 T.
-constexpr int row_values(int T) { return T >= 1024 ? 4 : kMaxV; }
+// 1024-thread workgroups keep at most 12, so the values live across the
+// transforms fit their 128-register budget without spilling (16 spilled);
+// the host's shape check admits a launch only when (K) nt W <= row_values T.
+// Tiles of at most kLeanValues cells per thread run a variant sized for
+// them, which issues fewer idle loop iterations.
+constexpr int row_values(int T) { return T >= 1024 ? 12 : kMaxV; }
+constexpr int kLeanValues = 4;
 
 __device__ __forceinline__ int tile_of(int ti, int nTiles) {
     return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
@@ -151,13 +154,13 @@ __device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, c
 // EXTRAS: the noise / muJitter / inject terms are compiled in (with all three
 // amounts zero both variants compute the same bits).
 // LDS: twiddles [W] | KPAR: Z [K][nt][W] ; serial: Z [nt][W] (sums in registers)
-template <int MODE, int T, bool EXTRAS, bool KPAR>
+template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T)>
 __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles,
                                                 uint32_t stepOff, int kSel,
                                                 float *__restrict__ out) {
     extern __shared__ float2 smem[];
-    constexpr int kV = row_values(T), kG = kV / 2 + 1;
+    constexpr int kV = V, kG = kV / 2 + 1;
     const int tid = threadIdx.x;
     const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
     if (ti < nTiles) do {
@@ -485,7 +488,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
 }
 
 // ---------------------------------------------------------------- launch templates
-template <int T, bool EX, bool KP>
+template <int T, bool EX, bool KP, int V = row_values(T)>
 void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
                         int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
                         hipStream_t s) {
@@ -493,7 +496,7 @@ void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTi
         launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
                      nTiles, stepOff, kSel, out);
     else if (mode == 1)
-        launch_timed(k_fft_rows<1, T, EX, KP>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
+        launch_timed(k_fft_rows<1, T, EX, KP, V>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
                      stepOff, kSel, out);
     else
         launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
@@ -533,7 +536,7 @@ LENIA_FFT_PASS_DECL(64)
 LENIA_FFT_PASS_DECL(128)
 LENIA_FFT_PASS_DECL(256)
 LENIA_FFT_PASS_DECL(512)
-void fft_rows_pass_1024(int mode, bool extras, const FftWorldDev *worlds, const int2 *tiles,
+void fft_rows_pass_1024(int mode, bool extras, bool lean, const FftWorldDev *worlds, const int2 *tiles,
                         int nTiles, int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
                         hipStream_t s);
 

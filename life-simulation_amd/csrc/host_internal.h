@@ -70,6 +70,8 @@ struct lenia_world {
         return desc.noiseAmount > 0.0f || desc.muJitter > 0.0f || desc.injectAmount > 0.0f;
     }
     float *current_state() const { return (isFft || parity() == 0) ? A0 : A1; }
+    // complex values one row-pass step tile holds in LDS: (K) x nt x W
+    int row_tile_cells() const { return (rowKpar ? (int)desc.nKernels : 1) * P * (int)desc.width; }
 };
 
 namespace lenia {

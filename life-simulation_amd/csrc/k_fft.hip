@@ -72,7 +72,7 @@ __global__ void k_spec_cols(const double *__restrict__ Cw, int h, int H, int Wh,
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                            size_t lds, uint32_t stepOff, int kSel, float *out,
-                           hipStream_t s) {
+                           hipStream_t s, int tileCells) {
     if (grid < nTiles) return hipErrorInvalidValue;  // one tile per workgroup
     switch (threads) {
         case 64: fft_rows_pass_64(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
@@ -81,7 +81,8 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
         case 512: fft_rows_pass_512(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 1024:  // steps with side-by-side transforms only (the serial form needs more registers)
             if (mode == 1 && !kpar) return hipErrorInvalidValue;
-            fft_rows_pass_1024(mode, extras, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+            fft_rows_pass_1024(mode, extras, tileCells > 0 && tileCells <= kLeanValues * 1024, worlds,
+                               tiles, nTiles, grid, lds, stepOff, kSel, out, s);
             break;
         default: return hipErrorInvalidValue;
     }

@@ -31,10 +31,13 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_
 // FFT path (k_fft.hip).  mode 0: forward row FFT of the state; 1: full step
 // row pass; 2: potential of kernel kSel into `out`.  The step kernels run the
 // step `stepOff` steps after each world's stored clock (mode 1 / mode 0 of
-// the direct forms).
+// the direct forms).  tileCells: the largest (K) x nt x W over the launch's
+// tiles when the caller knows it (lets 1024-thread steps run the variant
+// sized for small tiles), else 0.
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
-                           size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s);
+                           size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s,
+                           int tileCells = 0);
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
                            int nTiles, int grid, size_t lds, hipStream_t s);
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s);

@@ -302,7 +302,7 @@ static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
                             w->ldsCols, s);
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, false, w->rowKpar, recs + j, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, 0, 0, nullptr, s);
+                                w->nRowTiles, w->ldsRows, 0, 0, nullptr, s, w->row_tile_cells());
     }
     return e;
 }

@@ -946,7 +946,8 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
                                 (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
+                                w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream,
+                                w->row_tile_cells());
     } else if (w->isMfma) {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,

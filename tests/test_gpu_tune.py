@@ -89,14 +89,26 @@ def test_row_pass_workgroup_sizes_same_bits(ctx):
 
 @pytest.mark.gpu
 def test_row_pass_1024_register_cap(ctx):
-    """1024-thread row passes hold at most 4 cells per thread: a tile of two
-    row pairs of the 1024-wide K=3 world (3 x 2 x 1024 > 4 x 1024) is refused
-    with an error, not run with cells left out"""
+    """1024-thread row passes hold at most 12 cells per thread, and tiles of at
+    most 4 run a variant sized for them.  A 2048-wide world with four kernels
+    and two row pairs per tile (4 x 2 x 2048 > 12 x 1024, within the LDS) is
+    refused with an error, not run with cells left out; one row pair per tile
+    (the 12-cell variant) and a 4096-wide world with three kernels (12 x 1024
+    exactly) give the same bytes as 512-thread workgroups"""
     import lenia
+    from lenia_cases import ring
 
-    d = dict(config1(), rowThreads=1024, rowPairsPerTile=2)
+    k4 = [dict(k) for k in config1()["kernels"]] + [ring(24, [1.0], 0.2, 0.02, 0.1)]
+    wide4 = dict(config1(2048, 128), kernels=k4, initRect=(0, 0, 0, 0))
     with pytest.raises(lenia.LeniaError, match="row-pass shape"):
-        lenia.World(ctx, d)
-    d = dict(config1(), rowThreads=512, rowPairsPerTile=2)
-    with lenia.World(ctx, d) as w:
-        assert w.resolved()["rowThreads"] == 512
+        lenia.World(ctx, dict(wide4, rowThreads=1024, rowPairsPerTile=2))
+    for d in (wide4, dict(config1(4096, 64), initRect=(0, 0, 0, 0))):
+        out = {}
+        for rt in (512, 1024):
+            with lenia.World(ctx, dict(d, rowThreads=rt, rowPairsPerTile=1)) as w:
+                assert w.resolved()["rowThreads"] == rt
+                w.reset(7777)
+                w.frame(0, 1)
+                w.step(6)
+                out[rt] = w.read().tobytes()
+        assert out[512] == out[1024], d["width"]
