@@ -46,6 +46,7 @@ constexpr int kMaxV = 16;  // complex values per thread per stage
 // them, which issues fewer idle loop iterations.
 constexpr int row_values(int T) { return T >= 1024 ? 12 : kMaxV; }
 constexpr int kLeanValues = 4;
+constexpr int kLeanCols = 8;
 
 __device__ __forceinline__ int tile_of(int ti, int nTiles) {
     return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
@@ -362,7 +363,9 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
 
 // ---------------------------------------------------------------- column pass
 // LDS: twiddles [H] | X [C][H] | KPAR: Y [K][C][H]
-template <int T, bool KPAR>
+// V: column values per thread held in registers (kMaxV; a tile of at most
+// kLeanCols per thread runs the variant sized for it)
+template <int T, bool KPAR, int V = kMaxV>
 __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
@@ -386,15 +389,15 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
         const size_t sStride = (size_t)Wh * H;
         LENIA_GLOBAL float2 *G0 = gp(reinterpret_cast<float2 *>(wd.G)) + (size_t)c0 * H;
         {
-            float2 fv[kMaxV];
+            float2 fv[V];
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < ne) fv[i] = gld2(F + e);
             }
             twc.store(tw, twg, H, tid);
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < ne) X[sw(e)] = fv[i];
             }
@@ -402,9 +405,9 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
         if (KPAR) {
             float2 *Y = X + lds_span(ne);
             const int nk = K * ne;
-            float sr[kMaxV];
+            float sr[V];
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < nk) {
                     int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
@@ -419,9 +422,9 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                 for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
             }
             // all reads of the forward spectrum first, then the scaled copies
-            float2 xv[kMaxV];
+            float2 xv[V];
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < nk) {
                     int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
@@ -429,7 +432,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                 }
             }
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
             }
@@ -442,38 +445,38 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
             __syncthreads();
             continue;
         }
-        float sr[kMaxV];
+        float sr[V];
 #pragma unroll
-        for (int i = 0; i < kMaxV; ++i) {
+        for (int i = 0; i < V; ++i) {
             int e = tid + i * T;
             if (e < ne) sr[i] = S0[e];
         }
         __syncthreads();
         fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
-        float2 xs[kMaxV];
+        float2 xs[V];
 #pragma unroll
-        for (int i = 0; i < kMaxV; ++i) {
+        for (int i = 0; i < V; ++i) {
             int e = tid + i * T;
             xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
         }
         if (wd.saveAhat) {
             LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < ne) gst2(sv + e, xs[i]);
             }
         }
         for (int k = 0; k < K; ++k) {
 #pragma unroll
-            for (int i = 0; i < kMaxV; ++i) {
+            for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
                 if (e < ne) X[sw(e)] = cscale(xs[i], sr[i]);
             }
             if (k + 1 < K) {
                 const LENIA_GLOBAL float *Sn = S0 + (size_t)(k + 1) * sStride;
 #pragma unroll
-                for (int i = 0; i < kMaxV; ++i) {
+                for (int i = 0; i < V; ++i) {
                     int e = tid + i * T;
                     if (e < ne) sr[i] = Sn[e];
                 }
@@ -517,9 +520,12 @@ void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
 }
 
 template <int T>
-void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                          int grid, size_t lds, hipStream_t s) {
-    if (kpar)
+void cols_dispatch(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,
+                   int nTiles, int grid, size_t lds, hipStream_t s) {
+    if (kpar && lean)
+        launch_timed(k_fft_cols<T, true, kLeanCols>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles);
+    else if (kpar)
         launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
     else
         launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
@@ -530,8 +536,8 @@ void cols_dispatch(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int 
     void fft_rows_pass_##T(int mode, bool extras, bool kpar, const FftWorldDev *worlds,         \
                            const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff, \
                            int kSel, float *out, hipStream_t s);                                \
-    void fft_cols_pass_##T(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,  \
-                           int grid, size_t lds, hipStream_t s);
+    void fft_cols_pass_##T(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,   \
+                           int nTiles, int grid, size_t lds, hipStream_t s);
 LENIA_FFT_PASS_DECL(64)
 LENIA_FFT_PASS_DECL(128)
 LENIA_FFT_PASS_DECL(256)

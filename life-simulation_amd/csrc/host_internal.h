@@ -72,6 +72,8 @@ struct lenia_world {
     float *current_state() const { return (isFft || parity() == 0) ? A0 : A1; }
     // complex values one row-pass step tile holds in LDS: (K) x nt x W
     int row_tile_cells() const { return (rowKpar ? (int)desc.nKernels : 1) * P * (int)desc.width; }
+    // complex values one column-pass tile holds in registers: (K) x nt x H
+    int col_tile_cells() const { return (colKpar ? (int)desc.nKernels : 1) * C * (int)desc.height; }
 };
 
 namespace lenia {

@@ -90,13 +90,14 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
 }
 
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
-                           int nTiles, int grid, size_t lds, hipStream_t s) {
+                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells) {
     if (grid < nTiles) return hipErrorInvalidValue;  // one tile per workgroup
+    const bool lean = tileCells > 0 && tileCells <= kLeanCols * threads;
     switch (threads) {
-        case 64: fft_cols_pass_64(kpar, worlds, tiles, nTiles, grid, lds, s); break;
-        case 128: fft_cols_pass_128(kpar, worlds, tiles, nTiles, grid, lds, s); break;
-        case 256: fft_cols_pass_256(kpar, worlds, tiles, nTiles, grid, lds, s); break;
-        case 512: fft_cols_pass_512(kpar, worlds, tiles, nTiles, grid, lds, s); break;
+        case 64: fft_cols_pass_64(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 128: fft_cols_pass_128(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 256: fft_cols_pass_256(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 512: fft_cols_pass_512(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -9,9 +9,9 @@ void fft_rows_pass_256(int mode, bool extras, bool kpar, const FftWorldDev *worl
     rows_dispatch<256>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
 }
 
-void fft_cols_pass_256(bool kpar, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                       int grid, size_t lds, hipStream_t s) {
-    cols_dispatch<256>(kpar, worlds, tiles, nTiles, grid, lds, s);
+void fft_cols_pass_256(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,
+                       int nTiles, int grid, size_t lds, hipStream_t s) {
+    cols_dispatch<256>(kpar, lean, worlds, tiles, nTiles, grid, lds, s);
 }
 
 }  // namespace lenia

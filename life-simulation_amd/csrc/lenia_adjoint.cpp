@@ -299,7 +299,7 @@ static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
     hipError_t e = load_state(w, ws->ckpt + ws->n * sg, s);
     for (uint32_t j = 0; j < m && e == hipSuccess; ++j) {
         e = launch_fft_cols(w->colT, w->colKpar, recs + j, w->colTiles, w->nColTiles, w->nColTiles,
-                            w->ldsCols, s);
+                            w->ldsCols, s, w->col_tile_cells());
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, false, w->rowKpar, recs + j, w->rowTiles, w->nRowTiles,
                                 w->nRowTiles, w->ldsRows, 0, 0, nullptr, s, w->row_tile_cells());

@@ -941,7 +941,8 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
     if (w->isFft) {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles,
-                            w->nColTiles, w->nColTiles, w->ldsCols, ctx->stream);
+                            w->nColTiles, w->nColTiles, w->ldsCols, ctx->stream,
+                            w->col_tile_cells());
         if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
             e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,

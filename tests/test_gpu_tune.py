@@ -112,3 +112,22 @@ def test_row_pass_1024_register_cap(ctx):
                 w.step(6)
                 out[rt] = w.read().tobytes()
         assert out[512] == out[1024], d["width"]
+
+
+@pytest.mark.gpu
+def test_column_pass_lean_variant_same_bits(ctx):
+    """column tiles of at most 8 values per thread run a variant sized for
+    them (3 x 512 cells at 256 threads); 64-thread tiles of the same world run
+    the full-size variant: both give the same bytes"""
+    import lenia
+
+    out = {}
+    for ct in (64, 256):
+        d = dict(config1(), colThreads=ct, colsPerTile=1)
+        with lenia.World(ctx, d) as w:
+            assert w.resolved()["colThreads"] == ct
+            w.reset(7777)
+            w.frame(0, 1)
+            w.step(12)
+            out[ct] = w.read().tobytes()
+    assert out[64] == out[256]
