@@ -46,7 +46,7 @@ constexpr int kMaxV = 16;  // complex values per thread per stage
 // them, which issues fewer idle loop iterations.
 constexpr int row_values(int T) { return T >= 1024 ? 12 : kMaxV; }
 constexpr int kLeanValues = 4;
-// 512-thread step tiles of at most 12 cells per thread run a 12-cell variant
+// 256- and 512-thread step tiles of at most 12 cells per thread run a 12-cell variant
 constexpr int kMidValues = 12;
 constexpr int kLeanCols = 8;
 
@@ -544,7 +544,9 @@ LENIA_FFT_PASS_DECL(64)
 LENIA_FFT_PASS_DECL(128)
 LENIA_FFT_PASS_DECL(256)
 LENIA_FFT_PASS_DECL(512)
-// 512-thread step pass holding kMidValues cells per thread (side-by-side transforms)
+// 256- / 512-thread step passes holding kMidValues cells per thread (side-by-side transforms)
+void fft_rows_step_256_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                           int grid, size_t lds, uint32_t stepOff, hipStream_t s);
 void fft_rows_step_512_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
                            int grid, size_t lds, uint32_t stepOff, hipStream_t s);
 void fft_rows_pass_1024(int mode, bool extras, bool lean, const FftWorldDev *worlds, const int2 *tiles,

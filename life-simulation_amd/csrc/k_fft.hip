@@ -77,7 +77,12 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
     switch (threads) {
         case 64: fft_rows_pass_64(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 128: fft_rows_pass_128(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
-        case 256: fft_rows_pass_256(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 256:
+            if (mode == 1 && kpar && tileCells > 0 && tileCells <= kMidValues * 256)
+                fft_rows_step_256_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s);
+            else
+                fft_rows_pass_256(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+            break;
         case 512:
             if (mode == 1 && kpar && tileCells > 0 && tileCells <= kMidValues * 512)
                 fft_rows_step_512_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s);

@@ -27,6 +27,8 @@ struct lenia_batch {
     bool extras = false;
     int rowT = 256, colT = 256;
     bool rowKpar = true, colKpar = true;
+    // largest (K) x nt x W (rows) and (K) x nt x H (columns) over the tiles
+    int rowCells = 0, colCells = 0;
     void *devWorlds = nullptr;
     int2 *rowTiles = nullptr, *colTiles = nullptr;
     int nRowTiles = 0, nColTiles = 0;
@@ -116,6 +118,7 @@ static int batch_setup_fft(lenia_batch *b) {
 
     std::vector<FftWorldDev> table(n);
     std::vector<int2> rt, ct;
+    b->rowCells = b->colCells = 0;
     for (int i = 0; i < n; ++i) {
         lenia_world *w = b->worlds[i];
         FftWorldDev f = w->fdev;  // same buffers, plans, clock; batch tile shape
@@ -132,6 +135,8 @@ static int batch_setup_fft(lenia_batch *b) {
         for (int t = 0; t < f.colTiles; ++t) ct.push_back(make_int2(i, t));
         b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar), lds_rows(f.W, f.P, 1, true)});
         b->ldsCols = std::max(b->ldsCols, lds_cols(f.H, f.C, K, b->colKpar));
+        b->rowCells = std::max(b->rowCells, (b->rowKpar ? K : 1) * f.P * f.W);
+        b->colCells = std::max(b->colCells, (b->colKpar ? K : 1) * f.C * f.H);
     }
     b->nRowTiles = (int)rt.size();
     b->nColTiles = (int)ct.size();
@@ -226,12 +231,12 @@ static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint3
     if (b->isFft) {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds, b->colTiles,
-                            b->nColTiles, b->nColTiles, b->ldsCols, ctx->stream);
+                            b->nColTiles, b->nColTiles, b->ldsCols, ctx->stream, b->colCells);
         if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
             e = launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, (const FftWorldDev *)b->devWorlds,
                                 b->rowTiles, b->nRowTiles, b->nRowTiles, b->ldsRows, off, 0,
-                                nullptr, ctx->stream);
+                                nullptr, ctx->stream, b->rowCells);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
