@@ -90,7 +90,9 @@ typedef struct lenia_world_desc {
      * encodes 32.  Tiles: row pairs per row-pass workgroup, columns per
      * column-pass workgroup, and the two workgroup sizes (64, 128, 256 or
      * 512; the row pass also 1024 when its kernel transforms run side by
-     * side and K x rowPairsPerTile x width <= 12288). */
+     * side and K x rowPairsPerTile x width <= 12288).  Side-by-side row
+     * transforms also need K x rowPairsPerTile <= rowThreads; a shape that
+     * fits no variant is rejected with LENIA_ERR_UNSUPPORTED. */
     uint32_t fftPlanW, fftPlanH;
     uint32_t rowPairsPerTile, colsPerTile;
     uint32_t rowThreads, colThreads;

@@ -390,7 +390,8 @@ static int ensure_graphs(lenia_world *w) {
 
 extern "C" int lenia_rollout_forward(lenia_world *w, uint32_t T, uint32_t ckptEvery, const float *A0,
                                      float *AT, int onDevice) {
-    if (!w || !A0 || !AT || T == 0) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_forward: bad argument");
+    if (!w || !A0 || !AT) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_forward: NULL argument");
+    if (T == 0) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_forward: T must be at least 1");
     int rc = check_differentiable(w);
     if (rc) return rc;
     lenia_ctx *ctx = w->ctx;
@@ -417,8 +418,8 @@ extern "C" int lenia_rollout_forward(lenia_world *w, uint32_t T, uint32_t ckptEv
 extern "C" int lenia_rollout_backward(lenia_world *w, uint32_t T, uint32_t ckptEvery,
                                       const float *A0, const float *gAT, float *gA0,
                                       lenia_param_grads *grads, int onDevice) {
-    if (!w || !gAT || T == 0)
-        return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_backward: bad argument");
+    if (!w || !gAT) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_backward: NULL argument");
+    if (T == 0) return fail(LENIA_ERR_INVALID_ARG, "lenia_rollout_backward: T must be at least 1");
     int rc = check_differentiable(w);
     if (rc) return rc;
     lenia_ctx *ctx = w->ctx;

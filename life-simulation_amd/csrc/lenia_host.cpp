@@ -544,6 +544,10 @@ bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpa
     const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
     if (lds > kFftLdsMax) return false;
     if (rows && (long)(kpar ? nt * K : nt) * p.n > (long)T * fft_row_values(T)) return false;
+    // side-by-side gather: each thread loads V/2 + 1 spectrum slots, and the
+    // tile has K nt (W/2 + 1).  Every launched variant has K nt W <= V T, so
+    // K nt <= T makes K nt (W/2 + 1) <= T (V/2 + 1): no slot is left unloaded.
+    if (rows && kpar && nt * K > T) return false;
     return plan_fits(p, kpar ? nt * K : nt, T, mv) && plan_fits(p, nt, T, mv);
 }
 

@@ -490,6 +490,7 @@ void scene_release(lenia_scene *s) {
 
 extern "C" int lenia_scene_describe(const char *json, size_t len, uint32_t module, lenia_world_desc *desc,
                                     lenia_colormap *cmap, char *name, size_t nameCap) {
+    if (!json || !desc) return fail(LENIA_ERR_INVALID_ARG, "lenia_scene_describe: NULL argument");
     json::Value j;
     int rc = parse_text(json, len, j);
     if (rc) return rc;

@@ -30,9 +30,15 @@
  *               circular convolution golden vectors; direct == FFT
  *   FFT         numpy.fft golden spectra; delta/constant/cosine closed forms;
  *               Parseval; round trip
- *   growth      uniform-field closed form A' = clip(c + dt(2 bell(c)-1));
- *               zero field stays zero; numpy golden steps
- *   gradient    FP64 central finite differences of lo_rollout
+ *   growth      uniform-field closed forms, mode 0 A' = clip(c + dt(2 bell(c)-1))
+ *               and mode 1 A' = clip(c + dt(sum w bell(c) / sum w - c)) (A' = c
+ *               when sum w = 0); zero field stays zero; numpy golden steps
+ *   gradient    torch float64 autograd golden vectors; FP64 central finite
+ *               differences of lo_rollout (small worlds) and, at the 256x256
+ *               T=64 size, of lo_rollout_piece (clip states of the recorded
+ *               trajectory held fixed, so no difference straddles a clip kink)
+ *   piece       lo_rollout_piece records the same trajectory as lo_rollout
+ *               and replaying its own record reproduces it bit for bit
  *   stamps      one stamp is the organism whole in one of the 8 dihedral
  *               orientations (all 8 occur over seeds); unrotated it is the
  *               organism as stored; overlapping stamps take the maximum
@@ -480,8 +486,8 @@ void lo_potential_fft(int W, int H, const double *A, const double *img, double *
 /* ------------------------------------------------------------------ */
 static double metal_round(double v) { return v < 0.0 ? -floor(-v + 0.5) : floor(v + 0.5); }
 
-void lo_growth(const lo_world *w, const double *A, const double *const *u, uint32_t seed,
-               uint32_t frameIndex, double *out) {
+static void growth_impl(const lo_world *w, const double *A, const double *const *u,
+                        uint32_t seed, uint32_t frameIndex, double *out, int clip) {
     const int W = w->width, H = w->height, K = w->nKernels;
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < H; ++y) {
@@ -531,9 +537,14 @@ void lo_growth(const lo_world *w, const double *A, const double *const *u, uint3
                     a2 = a2 + (0.6 - a2) * t;
                 }
             }
-            out[i] = a2 < 0.0 ? 0.0 : (a2 > 1.0 ? 1.0 : a2);
+            out[i] = clip ? (a2 < 0.0 ? 0.0 : (a2 > 1.0 ? 1.0 : a2)) : a2;
         }
     }
+}
+
+void lo_growth(const lo_world *w, const double *A, const double *const *u, uint32_t seed,
+               uint32_t frameIndex, double *out) {
+    growth_impl(w, A, u, seed, frameIndex, out, 1);
 }
 
 /* ------------------------------------------------------------------ */
@@ -602,6 +613,43 @@ int lo_rollout(const lo_world *w, const double *A0, int T, int use_fft, double *
         potentials(w, &p, cur, u, use_fft);
         lo_growth(w, cur, (const double *const *)u, 0u, 0u, AT);
         memcpy(cur, AT, sizeof(double) * n);
+    }
+    memcpy(AT, cur, sizeof(double) * n);
+    free(cur);
+    for (int k = 0; k < w->nKernels; ++k) free(u[k]);
+    unprepare(w, &p);
+    return 0;
+}
+
+/* T-step rollout on one smooth piece of the clipped map (test helper, */
+/* no reference counterpart).  record != 0: the plain rollout, storing  */
+/* each step's clip state in mask[t*n + i] (0 inside, 1 clipped to 0,   */
+/* 2 clipped to 1).  record == 0: the same steps with those states      */
+/* imposed (0 -> the unclipped pre-clip value, 1 -> 0, 2 -> 1), i.e. the */
+/* smooth branch the recorded trajectory lies on; finite differences of */
+/* it are free of clip kinks, and equal the gradient the adjoint takes. */
+int lo_rollout_piece(const lo_world *w, const double *A0, int T, unsigned char *mask, int record,
+                     double *AT) {
+    if (w->nKernels < 1 || w->nKernels > 4) return -1;
+    size_t n = (size_t)w->width * w->height;
+    prepared p;
+    prepare(w, &p, 1);
+    double *u[4];
+    for (int k = 0; k < w->nKernels; ++k) u[k] = (double *)malloc(sizeof(double) * n);
+    double *cur = (double *)malloc(sizeof(double) * n);
+    memcpy(cur, A0, sizeof(double) * n);
+    for (int t = 0; t < T; ++t) {
+        unsigned char *m = mask + n * (size_t)t;
+        potentials(w, &p, cur, u, 1);
+        growth_impl(w, cur, (const double *const *)u, 0u, 0u, AT, 0);
+        for (size_t i = 0; i < n; ++i) {
+            if (record) {
+                m[i] = AT[i] <= 0.0 ? 1 : (AT[i] >= 1.0 ? 2 : 0);
+                cur[i] = AT[i] < 0.0 ? 0.0 : (AT[i] > 1.0 ? 1.0 : AT[i]);
+            } else {
+                cur[i] = m[i] == 1 ? 0.0 : (m[i] == 2 ? 1.0 : AT[i]);
+            }
+        }
     }
     memcpy(AT, cur, sizeof(double) * n);
     free(cur);

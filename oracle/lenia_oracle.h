@@ -94,6 +94,11 @@ int lo_step(const lo_world *w, const double *A, double *out, int use_fft, uint32
 int lo_rollout(const lo_world *w, const double *A0, int T, int use_fft, double *AT);
 int lo_rollout_grad(const lo_world *w, const double *A0, int T, const double *gT, double *gA0,
                     double *gMu, double *gSigma, double *gWeight, double *gBetas, double *gDt);
+/* FFT-form rollout on one smooth piece of the clipped map (test helper): record != 0 stores
+ * each step's clip state (0 inside, 1 low, 2 high) in mask[T*W*H]; record == 0 replays the
+ * steps with those states imposed, so central differences never straddle a clip kink */
+int lo_rollout_piece(const lo_world *w, const double *A0, int T, unsigned char *mask, int record,
+                     double *AT);
 
 /* ---- colour layer and activity ---- */
 /* cosine palette of colorMapField / colorMapFieldScaled (Shaders/Render/ColorMap.metal:17-75):

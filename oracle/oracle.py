@@ -108,6 +108,9 @@ def lib():
         L.lo_rollout_grad.restype = C.c_int
         L.lo_rollout_grad.argtypes = [C.POINTER(LoWorld), f64p, C.c_int, f64p, f64p,
                                       f64p, f64p, f64p, f64p, f64p]
+        L.lo_rollout_piece.restype = C.c_int
+        L.lo_rollout_piece.argtypes = [C.POINTER(LoWorld), f64p, C.c_int, C.POINTER(C.c_uint8),
+                                       C.c_int, f64p]
         L.lo_init_stamp.argtypes = [C.c_int, C.c_int, f64p, C.c_int, C.c_int, u32, u32, u32, f64p]
         L.lo_colormap.argtypes = [f64p, C.c_int, C.c_int, f64p, f64p, f64p, f64p, C.c_double,
                                   C.c_double, C.c_double, C.c_int, f64p, C.c_int, C.c_int]
@@ -317,6 +320,24 @@ def rollout(world: dict, A0, T: int, use_fft=True) -> np.ndarray:
     if rc != 0:
         raise ValueError("oracle rollout rejected the world description")
     return out
+
+
+def rollout_piece(world: dict, A0, T: int, mask=None):
+    """(A_T, mask): with mask None, the FFT-form rollout recording each step's
+    clip state; given a recorded mask, the rollout with those clip states
+    imposed (the smooth piece of the map the recorded trajectory lies on)"""
+    s = world_struct(world)
+    A0 = _c64(A0)
+    out = np.zeros_like(A0)
+    record = mask is None
+    m = np.zeros((T,) + A0.shape, np.uint8) if record else np.ascontiguousarray(mask, np.uint8)
+    if m.shape != (T,) + A0.shape:
+        raise ValueError("mask shape does not match (T, H, W)")
+    rc = lib().lo_rollout_piece(C.byref(s), _p(A0), T, m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                int(record), _p(out))
+    if rc != 0:
+        raise ValueError("oracle rollout rejected the world description")
+    return out, m
 
 
 def rollout_grad(world: dict, A0, T: int, gT) -> dict:

@@ -78,3 +78,62 @@ def test_sweep_worlds_are_valid():
         R = w["kernels"][0]["radius"]
         assert 2 * R + 1 <= min(w["width"], w["height"])
         assert w["dt"] in (0.05, 0.1, 0.2)
+
+
+def _build_c_caller(tmp_path):
+    import shutil
+    import subprocess
+
+    from lenia import _abi
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lib = _abi.LIB_PATH
+    exe = str(tmp_path / "abi_caller")
+    subprocess.run([cc, "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "abi_caller.c"), "-o", exe,
+                    lib, "-Wl,-rpath," + os.path.dirname(lib)], check=True)
+    return exe
+
+
+def test_c_caller_against_the_header(tmp_path):
+    """A plain C program compiled against include/lenia_mi355x.h: the header's
+    sizeof / offsetof equal the Python mirror's; every NULL-argument call
+    returns LENIA_ERR_INVALID_ARG and leaves a message for lenia_last_error;
+    lenia_scene_describe resolves the reference's multikernel preset through
+    the C ABI (no GPU call is made)."""
+    import subprocess
+
+    from lenia import _abi
+
+    exe = _build_c_caller(tmp_path)
+    scene = os.path.join(ROOT, "tests", "golden", "scenes", "lenia_multikernel.json")
+    out = subprocess.run([exe, scene], check=True, capture_output=True, text=True).stdout
+    mirrors = {"lenia_kernel_desc": _abi.KernelDesc, "lenia_world_desc": _abi.WorldDesc,
+               "lenia_param_grads": _abi.ParamGrads, "lenia_colormap": _abi.ColorMap,
+               "lenia_scene_info": _abi.SceneInfo}
+    seen = {k: set() for k in mirrors}
+    for line in out.splitlines():
+        f = line.split()
+        if f[0] == "size":
+            assert C.sizeof(mirrors[f[1]]) == int(f[2]), line
+        elif f[0] == "offset":
+            assert getattr(mirrors[f[1]], f[2]).offset == int(f[3]), line
+            seen[f[1]].add(f[2])
+    for name, m in mirrors.items():
+        assert seen[name] == {fld for fld, _ in m._fields_}, name
+    rcs = [ln for ln in out.splitlines() if ln.startswith("rc ")]
+    assert len(rcs) == 11
+    for ln in rcs:
+        assert f" {_abi.ERR_INVALID_ARG} " in ln, ln
+        assert "NULL" in ln or "null" in ln.lower(), ln
+    assert "gfx950" in out
+    desc = [ln for ln in out.splitlines() if ln.startswith("describe")][0].split()
+    # Presets/lenia_multikernel.json: simWidth/simHeight 1024x512, FFT, growth mode 1,
+    # three kernels, radius 24, dt 0.35
+    assert desc[2] == "0" and desc[4] == "lenia0"
+    assert desc[6] == "1024x512" and desc[8] == "1" and desc[10] == "1" and desc[12] == "3"
+    assert desc[14] == "24" and desc[16] == "0.350"
+    kern = [ln.split() for ln in out.splitlines() if ln.startswith("kernel ")]
+    assert [k[3] for k in kern] == ["24.000", "13.200", "7.200"]

@@ -175,6 +175,47 @@ def test_gradient_config_256_two_kernels(ctx):
     w.close()
 
 
+C4_GRAD_TOL = 1e-4  # observed 1.2e-6 (dA0 rel L2) and 4e-6 (parameters) on MI355X
+
+
+def test_gradient_config4_T64_ckpt8(ctx):
+    """The differentiable-rollout configuration at its stated size: 256x256,
+    two ring kernels R=16, T=64 steps with checkpoints every 8 (recomputed
+    segments), from one step after seeded full-frame noise, upstream gradient
+    of an MSE-like loss.  Against the FP64 oracle gradient (which
+    tests/test_oracle.py checks against FP64 central differences at this same
+    size): dA0 relative L2 and every parameter gradient relative to the
+    largest within 1e-4; the state at T is still alive (the gradient flows
+    through the clip mask of live cells)."""
+    import lenia
+
+    desc = dict(config4(), initMode=INIT_NONE)
+    A0 = seeded_state(ctx, desc, 1234)
+    rng = np.random.default_rng(5)
+    gT = rng.standard_normal(A0.shape).astype(np.float32)
+    T, c = 64, 8
+    with lenia.World(ctx, desc) as w:
+        AT = w.rollout(A0, T, ckpt_every=c)
+        g = w.rollout_grad(None, T, gT, ckpt_every=c)
+    live = float(((AT > 0) & (AT < 1)).mean())
+    print(f"256x256 T=64: {100 * live:.1f}% of cells live at T")
+    assert live > 0.05
+    ref = O.rollout_grad(desc, A0.astype(np.float64), T, gT.astype(np.float64))
+    e = rel_l2(g["A0"], ref["A0"])
+    print(f"256x256 T=64 ckpt=8: dA0 rel L2 {e:.3e}")
+    assert e <= C4_GRAD_TOL
+    scale = max(np.abs(np.concatenate([ref["mu"], ref["sigma"], ref["weight"]])).max(),
+                abs(ref["dt"]), np.abs(ref["betas"]).max())
+    for key in ("mu", "sigma", "weight", "betas"):
+        d = float(np.abs(g[key] - ref[key]).max() / scale)
+        print(f"256x256 T=64 ckpt=8: d{key} rel {d:.3e}  gpu {np.ravel(g[key])[:4]} "
+              f"fp64 {np.ravel(ref[key])[:4]}")
+        assert d <= C4_GRAD_TOL, key
+    d = abs(g["dt"] - ref["dt"]) / scale
+    print(f"256x256 T=64 ckpt=8: ddt rel {d:.3e}  gpu {g['dt']:.6f} fp64 {ref['dt']:.6f}")
+    assert d <= C4_GRAD_TOL
+
+
 def test_gradient_after_param_change(ctx):
     """set_params between calls: the reverse pass follows the new values"""
     import lenia

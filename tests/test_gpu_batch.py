@@ -153,3 +153,50 @@ def test_sweep_256_one_step_matches_oracle_sample(ctx):
     finally:
         for w in worlds:
             w.close()
+
+
+def test_sweep_256_as_benchmarked(ctx):
+    """The batched sweep exactly as it is benchmarked: all 256 worlds, launch
+    shapes from lenia_batch_autotune, 50-step graph replay, 250 steps (one
+    50-step warm-up sequence and 200 more).  Every world is byte-compared with
+    the same world stepped alone (its own default launch shapes and 32-step
+    graphs); then one more batched step from the final states is checked
+    against the FP64 oracle on a sample that favours the worlds still alive."""
+    import lenia
+    import oracle as O
+    from lenia_cases import live_cells
+
+    descs = sweep_worlds(256)
+    seeds = [1000 + i for i in range(256)]
+    worlds = [lenia.World(ctx, d) for d in descs]
+    try:
+        for w, s in zip(worlds, seeds):
+            w.reset(s)
+            w.frame(0, 1)
+        with lenia.Batch(ctx, worlds) as bt:
+            ms = bt.autotune(10)
+            bt.step(50, 50)
+            bt.step(200, 50)
+            got = [w.read() for w in worlds]
+            alive = [i for i, A in enumerate(got) if live_cells(A) > 0]
+            print(f"sweep 256: tuned {ms * 1e3:.1f} us per batched step; {len(alive)} of 256 "
+                  f"worlds alive after 250 steps")
+            for i, d in enumerate(descs):
+                with lenia.World(ctx, d) as w:
+                    w.reset(seeds[i])
+                    w.frame(0, 1)
+                    w.step(250)
+                    assert got[i].tobytes() == w.read().tobytes(), (i, d["width"], d["height"])
+            assert len(alive) >= 8
+            sample = alive[:: max(1, len(alive) // 8)][:8] + list(range(3, 256, 64))
+            bt.step(1)
+            for i in sample:
+                want = O.step(descs[i], got[i].astype(np.float64), use_fft=True)
+                mx = float(np.abs(worlds[i].read().astype(np.float64) - want).max())
+                print(f"sweep world {i} {descs[i]['width']}x{descs[i]['height']} "
+                      f"K={len(descs[i]['kernels'])} ({live_cells(got[i])} live): step 251 "
+                      f"max-abs {mx:.2e}")
+                assert mx <= 1e-5
+    finally:
+        for w in worlds:
+            w.close()

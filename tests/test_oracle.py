@@ -219,6 +219,42 @@ def test_uniform_field_closed_form():
         np.testing.assert_allclose(O.step(w, A, use_fft=True), want, atol=1e-14)
 
 
+def _mode1_world(weights):
+    ks = [dict(shape=O.SHAPE_RING, core=O.CORE_GAUSSIAN, radius=6.0, shellSigma=0.15,
+               betas=[1.0], mu=0.15, sigma=0.017, weight=weights[0]),
+          dict(shape=O.SHAPE_RING, core=O.CORE_GAUSSIAN, radius=4.0, shellSigma=0.15,
+               betas=[1.0, 0.6], mu=0.30, sigma=0.035, weight=weights[1]),
+          dict(shape=O.SHAPE_SHELL, core=O.CORE_EXP, radius=5.0, shellMu=0.5, shellSigma=0.15,
+               betas=[1.0], mu=0.22, sigma=0.028, weight=weights[2])]
+    return dict(width=32, height=24, growthMode=1, dt=0.35, radius=6, kernels=ks)
+
+
+def test_uniform_field_closed_form_mode1():
+    """growth mode 1 (Lenia.metal:95-100, 188-203): a uniform field c has
+    potential c under every kernel (each sums to 1), so
+    A' = clip(c + dt (sum_k w_k bell_k(c) / sum_k w_k - c)), and A' = c when
+    sum_k w_k <= 1e-6; both convolution forms"""
+    mus, sigmas = (0.15, 0.30, 0.22), (0.017, 0.035, 0.028)
+    for weights in ((1.0, 0.6, 0.4), (0.0, 0.0, 2.5), (0.0, 0.0, 0.0)):
+        w = _mode1_world(weights)
+        for c in (0.0, 0.1, 0.15, 0.22, 0.3, 0.5, 0.97, 1.0):
+            A = np.full((24, 32), c)
+            sw = sum(weights)
+            if sw > 1e-6:
+                tgt = sum(wk * np.exp(-0.5 * ((c - m) / sg) ** 2)
+                          for wk, m, sg in zip(weights, mus, sigmas)) / sw
+            else:
+                tgt = c
+            want = np.clip(c + 0.35 * (tgt - c), 0, 1)
+            np.testing.assert_allclose(O.step(w, A), want, atol=1e-15)
+            np.testing.assert_allclose(O.step(w, A, use_fft=True), want, atol=1e-14)
+    # the single-kernel leniaStep form A' = A + dt (bell - A)
+    w = dict(_mode1_world((1.0, 0.0, 0.0)), kernels=_mode1_world((1.0, 0, 0))["kernels"][:1])
+    for c in (0.12, 0.15, 0.4):
+        want = np.clip(c + 0.35 * (np.exp(-0.5 * ((c - 0.15) / 0.017) ** 2) - c), 0, 1)
+        np.testing.assert_allclose(O.step(w, np.full((24, 32), c)), want, atol=1e-15)
+
+
 def test_zero_field_stays_zero_mode0():
     w = as_oracle_world(WORLDS["w3_mode0"])
     A = np.zeros((28, 36))
@@ -300,6 +336,77 @@ def test_gradient_central_differences():
         assert abs(fd - g["A0"][y, x]) <= 1e-5 * max(1.0, abs(fd))
 
 
+def test_rollout_piece_replays_the_recorded_trajectory():
+    """the clip-frozen rollout: recording gives lo_rollout's trajectory, and
+    replaying the record from the same start reproduces it bit for bit"""
+    w = _grad_world()
+    rng = np.random.default_rng(2)
+    A0 = rng.random((20, 24))
+    A0[rng.random((20, 24)) < 0.3] = 0.0
+    AT, mask = O.rollout_piece(w, A0, 7)
+    assert np.array_equal(AT, O.rollout(w, A0, 7))
+    assert mask.shape == (7, 20, 24) and set(np.unique(mask)) <= {0, 1, 2}
+    assert np.count_nonzero(mask == 1) > 0  # the trajectory does clip
+    AT2, _ = O.rollout_piece(w, A0, 7, mask)
+    assert np.array_equal(AT2, AT)
+
+
+def test_gradient_central_differences_config4_size():
+    """The oracle gradient at the differentiable-rollout size (256x256, two
+    ring kernels R=16, T=64) against FP64 central differences, h = 1e-6.
+    A parameter moves every cell, so over 64 x 65536 cell-steps some pre-clip
+    value sits within reach of 0 or 1 and a plain difference straddles the
+    kink (mu: 43 flips, 0.8% off); the parameter probes therefore difference
+    lo_rollout_piece with the base trajectory's clip states held, which is the
+    smooth branch the adjoint differentiates.  State probes are local and
+    difference the plain rollout, after checking that no clip state flips."""
+    from lenia_cases import INIT_NONE, config4
+
+    w = dict(config4(), initMode=INIT_NONE)
+    A0 = O.step(w, O.init_patch(256, 256, O.module_seed(1234), 0, 0, 256, 256), use_fft=True)
+    gT = np.random.default_rng(5).standard_normal((256, 256))
+    T, h = 64, 1e-6
+    g = O.rollout_grad(w, A0, T, gT)
+    _, mask = O.rollout_piece(w, A0, T)
+
+    def piece_loss(wd):
+        return float(np.sum(O.rollout_piece(wd, A0, T, mask)[0] * gT))
+
+    def with_kernel(k, key, v):
+        wd = {**w, "kernels": [dict(kk) for kk in w["kernels"]]}
+        wd["kernels"][k][key] = v
+        return wd
+
+    probes = []
+    for k in range(2):
+        for key in ("mu", "sigma", "weight"):
+            v = w["kernels"][k][key]
+            probes.append((f"{key}[{k}]", with_kernel(k, key, v + h), with_kernel(k, key, v - h),
+                           g[key][k]))
+    probes.append(("dt", dict(w, dt=w["dt"] + h), dict(w, dt=w["dt"] - h), g["dt"]))
+    for j in range(2):
+        bp, bm = list(w["kernels"][1]["betas"]), list(w["kernels"][1]["betas"])
+        bp[j] += h
+        bm[j] -= h
+        probes.append((f"betas[1][{j}]", with_kernel(1, "betas", bp), with_kernel(1, "betas", bm),
+                       g["betas"][1][j]))
+    for name, wp, wm, ana in probes:
+        fd = (piece_loss(wp) - piece_loss(wm)) / (2 * h)
+        rel = abs(fd - ana) / max(1.0, abs(fd))
+        print(f"{name}: FD {fd:.9g} adjoint {ana:.9g} rel {rel:.2e}")
+        assert rel <= 1e-5, (name, fd, ana)
+    for (y, x) in [(40, 50), (128, 128), (200, 17)]:
+        Ap, Am = A0.copy(), A0.copy()
+        Ap[y, x] += h
+        Am[y, x] -= h
+        (ATp, mp), (ATm, mm) = O.rollout_piece(w, Ap, T), O.rollout_piece(w, Am, T)
+        assert np.array_equal(mp, mm), "state probe straddles a clip kink"
+        fd = float(np.sum((ATp - ATm) * gT)) / (2 * h)
+        rel = abs(fd - g["A0"][y, x]) / max(1.0, abs(fd))
+        print(f"A0[{y},{x}]: FD {fd:.9g} adjoint {g['A0'][y, x]:.9g} rel {rel:.2e}")
+        assert rel <= 1e-5
+
+
 def test_value_noise_init_invariants():
     A = O.init_value_noise(64, 48, 1234, 0.42, 26.0)
     assert A.min() >= 0.0 and A.max() <= 0.9
@@ -315,3 +422,22 @@ def test_patch_init():
     assert np.all(A[:, :10] == 0) and np.all(A[:, 30:] == 0)
     assert A[5, 10] == O.rand01(10, 5, O.TAG_PATCH, 99)
     assert 0.3 < A[5:21, 10:30].mean() < 0.7
+
+
+def test_liveness_guard_rejects_dead_uniform_and_static_fields():
+    """the guard that the multi-step GPU comparisons call before they compare
+    anything: an empty field, a uniform fixed point and an unchanged field
+    fail it; a structured evolving field passes"""
+    from lenia_cases import assert_alive
+
+    with pytest.raises(AssertionError, match="dead"):
+        assert_alive(np.zeros((64, 64)), min_live=10)
+    with pytest.raises(AssertionError, match="dead"):
+        assert_alive(np.ones((64, 64)), min_live=10)
+    with pytest.raises(AssertionError, match="uniform"):
+        assert_alive(np.full((64, 64), 0.275), min_live=10, min_std=0.05)
+    A = O.init_value_noise(64, 64, 1234, 0.42, 26.0)
+    with pytest.raises(AssertionError, match="static"):
+        assert_alive(A, A, min_live=10, min_change=1e-3)
+    assert A.std() > 0.01
+    assert assert_alive(A, 0.9 * A, min_live=10, min_std=0.01, min_change=1e-3) >= 10
