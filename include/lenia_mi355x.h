@@ -179,6 +179,25 @@ int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
  * step.  Direct forms: nothing to tune, *msPerStep = 0. */
 int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msPerStep);
 
+/* ---- allocation-free stepping (no counterpart in the reference): for callers
+ * that cannot allocate device memory -- code under stream capture, serving
+ * loops, harnesses that forbid hipMalloc.  Steps the caller's device field
+ * `state` (width*height floats, row-major, updated in place) nSteps times
+ * with the FFT form of `desc` on ctx's stream, step i at frame
+ * frameIndex + i, substep 0, with module seed deriveSeed(sceneSeed, 'LEN1'):
+ * the results equal lenia_world_step's on a world of the same descriptor,
+ * reset with sceneSeed, holding the same state at that frame.  Spectra,
+ * twiddles, tile lists and the device record live in a library-owned device
+ * array of 64 MiB per GPU (taken when the first context on a GPU is created,
+ * held until the process ends); no call allocates.  Fields whose buffers do
+ * not fit it (a 1920x1080 field with 3 kernels fits) are rejected with
+ * LENIA_ERR_INVALID_ARG.  Every call on a GPU shares that array: calls that
+ * overlap on one GPU (two streams or threads) race.  Asynchronous except
+ * when the descriptor differs from the previous call's on the GPU (the
+ * kernel spectra are then rebuilt, and the call waits for that). */
+int lenia_fft_step_scratch(lenia_ctx *ctx, const lenia_world_desc *desc, uint32_t sceneSeed,
+                           uint32_t frameIndex, float *state, uint32_t nSteps);
+
 /* ---- differentiable rollout (no counterpart in the reference; for callers
  * that fit parameters and initial states to target fields).  FFT form,
  * growthMode 0, no noise / jitter / inject.  Pointers are device pointers

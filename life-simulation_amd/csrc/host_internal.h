@@ -107,6 +107,13 @@ int world_ensure_spectrum(lenia_world *w);
 int world_apply_init(lenia_world *w);
 
 int world_upload_desc(lenia_world *w);
+int world_check_desc(const lenia_world_desc *desc);
+GrowthParams world_growth(const lenia_world_desc *desc, uint32_t moduleSeed);
+int fft_sizes_and_tiles(lenia_world *w, int numCUs);
+FftWorldDev fft_record(lenia_world *w);
+// the allocation-free step's per-device scratch address (lenia_scratch.cpp),
+// looked up once per device when a context is created on it
+int scratch_init_device(int device);
 int world_launch_steps(lenia_world *w, uint32_t n);
 void world_drop_graph(lenia_world *w);
 void world_free_adjoint(lenia_world *w);

@@ -87,6 +87,10 @@ extern "C" int lenia_ctx_create(int device, lenia_ctx **out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(LENIA_ERR_UNSUPPORTED, "lenia_ctx_create: device is %s, this build is gfx950 only",
                     prop.gcnArchName);
+    // the allocation-free step's per-device scratch address, looked up once
+    // per device with the device current
+    int rc = scratch_init_device(device);
+    if (rc) return rc;
     lenia_ctx *c = new lenia_ctx();
     c->device = device;
     c->numCUs = prop.multiProcessorCount;
@@ -614,35 +618,67 @@ static int choose_tiles_fft(lenia_world *w, int numCUs) {
     return LENIA_OK;
 }
 
+// Size rules and launch shapes of an FFT-form world (w->desc set): what
+// lenia_world_create and the scratch-backed step both resolve.
+int lenia::fft_sizes_and_tiles(lenia_world *w, int numCUs) {
+    const int W = w->desc.width, H = w->desc.height;
+    if (W % 2 || H % 2)
+        return fail(LENIA_ERR_UNSUPPORTED, "FFT form needs even width and height (got %dx%d)", W, H);
+    int a_, b_, c_;
+    if (!factor235(W, a_, b_, c_) || !factor235(H, a_, b_, c_))
+        return fail(LENIA_ERR_UNSUPPORTED, "FFT form needs sides of the form 2^a 3^b 5^c (got %dx%d)",
+                    W, H);
+    if (W > 4096 || H > 4096)
+        return fail(LENIA_ERR_UNSUPPORTED, "FFT form supports sides up to 4096 (got %dx%d)", W, H);
+    return choose_tiles_fft(w, numCUs);
+}
+
+int lenia::world_check_desc(const lenia_world_desc *desc) { return check_desc(desc); }
+
+GrowthParams lenia::world_growth(const lenia_world_desc *desc, uint32_t moduleSeed) {
+    return growth_of(desc, moduleSeed);
+}
+
+// The device record of an FFT-form world from its host fields (buffer
+// pointers included); also sets the tile counts and LDS sizes of its passes.
+FftWorldDev lenia::fft_record(lenia_world *w) {
+    FftWorldDev f;
+    std::memset(&f, 0, sizeof(f));
+    f.W = w->desc.width;
+    f.H = w->desc.height;
+    f.Wh = f.W / 2 + 1;
+    f.P = w->P;
+    f.C = w->C;
+    f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
+    f.colTiles = (f.Wh + f.C - 1) / f.C;
+    f.planW = w->planW;
+    f.planH = w->planH;
+    f.K = (int)w->desc.nKernels;
+    f.mP = div_magic(f.P);
+    f.mW = div_magic(f.W);
+    f.rowThreads = w->rowT;
+    f.colThreads = w->colT;
+    f.g = w->g;
+    f.twW = w->twW;
+    f.twH = w->twH;
+    f.A = w->A0;
+    f.F = w->F;
+    f.G = w->G;
+    f.S = w->S;
+    f.clock = w->clock;
+    w->fdev = f;
+    w->nRowTiles = f.rowTiles;
+    w->nColTiles = f.colTiles;
+    w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
+    w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
+    return f;
+}
+
 int lenia::world_upload_desc(lenia_world *w) {
     lenia_ctx *ctx = w->ctx;
     w->version += 1;
     if (w->isFft) {
-        FftWorldDev f;
-        std::memset(&f, 0, sizeof(f));
-        f.W = w->desc.width;
-        f.H = w->desc.height;
-        f.Wh = f.W / 2 + 1;
-        f.P = w->P;
-        f.C = w->C;
-        f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
-        f.colTiles = (f.Wh + f.C - 1) / f.C;
-        f.planW = w->planW;
-        f.planH = w->planH;
-        f.K = (int)w->desc.nKernels;
-        f.mP = div_magic(f.P);
-        f.mW = div_magic(f.W);
-        f.rowThreads = w->rowT;
-        f.colThreads = w->colT;
-        f.g = w->g;
-        f.twW = w->twW;
-        f.twH = w->twH;
-        f.A = w->A0;
-        f.F = w->F;
-        f.G = w->G;
-        f.S = w->S;
-        f.clock = w->clock;
-        w->fdev = f;
+        const FftWorldDev f = fft_record(w);
         HIP_TRY(hipMemcpyAsync(w->devDesc, &f, sizeof(f), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
         std::vector<int2> rt, ct;
@@ -652,10 +688,6 @@ int lenia::world_upload_desc(lenia_world *w) {
                                hipMemcpyHostToDevice, ctx->stream), "upload tiles");
         HIP_TRY(hipMemcpyAsync(w->colTiles, ct.data(), ct.size() * sizeof(int2),
                                hipMemcpyHostToDevice, ctx->stream), "upload tiles");
-        w->nRowTiles = f.rowTiles;
-        w->nColTiles = f.colTiles;
-        w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
-        w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
     } else if (w->isMfma) {
         MfmaWorldDev md;
         std::memset(&md, 0, sizeof(md));
@@ -741,21 +773,7 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
     w->isMfma = desc->convMode == LENIA_CONV_DIRECT_BF16 || desc->convMode == LENIA_CONV_DIRECT_BF16X3;
     const int W = desc->width, H = desc->height;
     if (w->isFft) {
-        if (W % 2 || H % 2) {
-            delete w;
-            return fail(LENIA_ERR_UNSUPPORTED, "FFT form needs even width and height (got %dx%d)", W, H);
-        }
-        int a_, b_, c_;
-        if (!factor235(W, a_, b_, c_) || !factor235(H, a_, b_, c_)) {
-            delete w;
-            return fail(LENIA_ERR_UNSUPPORTED,
-                        "FFT form needs sides of the form 2^a 3^b 5^c (got %dx%d)", W, H);
-        }
-        if (W > 4096 || H > 4096) {
-            delete w;
-            return fail(LENIA_ERR_UNSUPPORTED, "FFT form supports sides up to 4096 (got %dx%d)", W, H);
-        }
-        rc = choose_tiles_fft(w, ctx->numCUs);
+        rc = fft_sizes_and_tiles(w, ctx->numCUs);
         if (rc) {
             delete w;
             return rc;

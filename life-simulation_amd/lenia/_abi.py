@@ -43,6 +43,7 @@ EXPORTS = [
     "lenia_scene_step", "lenia_scene_set_param", "lenia_scene_get_info", "lenia_scene_world",
     "lenia_scene_layer", "lenia_sweep_create", "lenia_sweep_destroy", "lenia_sweep_count",
     "lenia_sweep_run", "lenia_sweep_activity", "lenia_sweep_world", "lenia_sweep_report",
+    "lenia_fft_step_scratch",
 ]
 
 
@@ -204,6 +205,7 @@ def lib():
                                        C.POINTER(C.c_int), u32]),
         "lenia_sweep_world": (i32, [vp, u32, C.POINTER(vp)]),
         "lenia_sweep_report": (i32, [vp, C.c_char_p, sz, C.POINTER(sz)]),
+        "lenia_fft_step_scratch": (i32, [vp, C.POINTER(WorldDesc), u32, u32, vp, u32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
