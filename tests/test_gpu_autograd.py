@@ -108,7 +108,8 @@ def test_adam_fit_config4(ctx):
     betas[1, 1] = 0.4
     betas.requires_grad_()
     A0 = (hA + 0.02 * noise).clamp(0, 1).requires_grad_()
-    start = {"mu": float((mu - hmu).abs().max()), "sigma": float((sigma - hsg).abs().max())}
+    start = {"mu": float((mu.detach() - hmu).abs().max()),
+             "sigma": float((sigma.detach() - hsg).abs().max())}
     opt = torch.optim.Adam([{"params": [mu, sigma, dt, weight, betas], "lr": 5e-4},
                             {"params": [A0], "lr": 2e-3}])
     losses = []
