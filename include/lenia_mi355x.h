@@ -118,8 +118,9 @@ int lenia_ctx_sync(lenia_ctx *ctx);
 int lenia_ctx_stream(lenia_ctx *ctx, void **stream);
 /* live device bytes and allocations (ResourcePool accounting, ResourcePool.h:78-81) */
 int lenia_ctx_memory(lenia_ctx *ctx, uint64_t *liveBytes, uint32_t *liveAllocs);
-/* GPU time in ms of the last lenia_world_step / lenia_batch_step call
- * (GPUTimer, LifeCore/Metal/GPUTimer.mm:9-94); waits for it */
+/* GPU time in ms of the last timed call -- lenia_world_frame / _step,
+ * lenia_batch_step / _frames, or lenia_scene_step (all its frames: world steps
+ * and colour layers) -- (GPUTimer, LifeCore/Metal/GPUTimer.mm:9-94); waits for it */
 int lenia_ctx_last_gpu_ms(lenia_ctx *ctx, double *ms);
 
 /* ---- world: replaces LeniaModule::setup (Lenia.cpp:208-408): allocates the

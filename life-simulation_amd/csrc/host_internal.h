@@ -14,7 +14,9 @@ struct lenia_ctx {
     int device = 0;
     int numCUs = 256;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around one world / batch call
+    hipEvent_t sp0 = nullptr, sp1 = nullptr;  // around a scene step (which makes world calls)
+    hipEvent_t last0 = nullptr, last1 = nullptr;  // the pair lenia_ctx_last_gpu_ms reads
     bool timed = false;
     uint64_t liveBytes = 0;
     uint32_t liveAllocs = 0;

@@ -426,6 +426,8 @@ static int batch_run_steps(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps)
     }
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->timed = true;
+    ctx->last0 = ctx->ev0;
+    ctx->last1 = ctx->ev1;
     for (lenia_world *w : b->worlds) w->hostSteps += nSteps;
     return LENIA_OK;
 }

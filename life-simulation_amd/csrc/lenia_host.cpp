@@ -101,6 +101,8 @@ extern "C" int lenia_ctx_create(int device, lenia_ctx **out) {
     }
     (void)hipEventCreate(&c->ev0);
     (void)hipEventCreate(&c->ev1);
+    (void)hipEventCreate(&c->sp0);
+    (void)hipEventCreate(&c->sp1);
     *out = c;
     return LENIA_OK;
 }
@@ -111,6 +113,8 @@ extern "C" int lenia_ctx_destroy(lenia_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->ev0);
     (void)hipEventDestroy(ctx->ev1);
+    (void)hipEventDestroy(ctx->sp0);
+    (void)hipEventDestroy(ctx->sp1);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LENIA_OK;
@@ -140,9 +144,9 @@ extern "C" int lenia_ctx_last_gpu_ms(lenia_ctx *ctx, double *ms) {
     if (!ctx || !ms) return fail(LENIA_ERR_INVALID_ARG, "lenia_ctx_last_gpu_ms: NULL argument");
     if (!ctx->timed) return fail(LENIA_ERR_STATE, "no timed step has run on this context");
     (void)hipSetDevice(ctx->device);
-    HIP_TRY(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+    HIP_TRY(hipEventSynchronize(ctx->last1), "hipEventSynchronize");
     float f = 0.f;
-    HIP_TRY(hipEventElapsedTime(&f, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+    HIP_TRY(hipEventElapsedTime(&f, ctx->last0, ctx->last1), "hipEventElapsedTime");
     *ms = f;
     return LENIA_OK;
 }
@@ -1069,6 +1073,8 @@ extern "C" int lenia_world_frame(lenia_world *w, uint32_t frameIndex, uint32_t s
     int rc = world_launch_steps(w, steps);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->timed = true;
+    ctx->last0 = ctx->ev0;
+    ctx->last1 = ctx->ev1;
     return rc;
 }
 
@@ -1087,6 +1093,8 @@ extern "C" int lenia_world_step(lenia_world *w, uint32_t nSteps) {
     int rc = world_launch_steps(w, nSteps);
     (void)hipEventRecord(ctx->ev1, ctx->stream);
     ctx->timed = true;
+    ctx->last0 = ctx->ev0;
+    ctx->last1 = ctx->ev1;
     return rc;
 }
 

@@ -637,10 +637,18 @@ static int scene_frame(lenia_scene *s) {
 extern "C" int lenia_scene_step(lenia_scene *s, uint32_t nFrames) {
     if (!s) return fail(LENIA_ERR_INVALID_ARG, "lenia_scene_step: NULL scene");
     (void)hipSetDevice(s->ctx->device);
+    lenia_ctx *ctx = s->ctx;
+    // GPU time of the whole call (the frames' world steps and colour layers),
+    // what LifeBench reports per frame (Apps/LifeBench/main.cpp:112-133)
+    (void)hipEventRecord(ctx->sp0, ctx->stream);
     for (uint32_t f = 0; f < nFrames; ++f) {
         int rc = scene_frame(s);
         if (rc) return rc;
     }
+    (void)hipEventRecord(ctx->sp1, ctx->stream);
+    ctx->timed = true;
+    ctx->last0 = ctx->sp0;
+    ctx->last1 = ctx->sp1;
     return LENIA_OK;
 }
 

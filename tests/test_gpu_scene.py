@@ -140,6 +140,47 @@ def test_multikernel_scene_matches_world(ctx):
     check_layer(layer, O.colormap(got.astype(np.float64), cm, 1280, 720), "multikernel layer", 5e-4)
 
 
+def test_multikernel_preset_300_frames_acceptance(ctx):
+    """The reference's acceptance run of the multikernel preset
+    (docs/specs/phase9_lenia_fft.md:163-164): Presets/lenia_multikernel.json
+    for 300 frames, twice -- every call succeeds, the final frame is not one
+    colour, and both runs give identical bytes (state and colour layer).  Then
+    frame 300 from the run's own state against the FP64 oracle (growth mode 1,
+    inject blobs at that frame's seed), within 1e-5."""
+    import hashlib
+
+    import lenia
+    from lenia_cases import live_cells
+
+    text = scene_text("lenia_multikernel.json")
+    runs = []
+    for _ in range(2):
+        with lenia.Scene(ctx, text) as sc:
+            sc.step(300)
+            state = sc.world(0).read()
+            layer = sc.layer(0)
+            runs.append((state, layer))
+            if len(runs) == 1:
+                prev = state
+                sc.step(1)
+                nxt = sc.world(0).read()
+    (s0, l0), (s1, l1) = runs
+    md5 = [hashlib.md5(l.tobytes()).hexdigest() for l in (l0, l1)]
+    rgb = l0[..., :3].astype(np.float64).reshape(-1, 3)
+    colours = len(np.unique(l0[..., :3].reshape(-1, 3), axis=0))
+    luma = 0.2126 * rgb[:, 0] + 0.7152 * rgb[:, 1] + 0.0722 * rgb[:, 2]
+    print(f"multikernel 300 frames: layer md5 {md5[0]} / {md5[1]}; {colours} distinct colours, "
+          f"luma mean {luma.mean():.3f} sd {luma.std():.3f}; {live_cells(s0)} live cells")
+    assert s0.tobytes() == s1.tobytes() and md5[0] == md5[1]
+    assert colours > 1 and luma.std() > 1e-3
+    d, _, _ = lenia.describe_scene(text)
+    m = O.module_seed(7777)
+    want = O.step(d, prev.astype(np.float64), use_fft=True, seed=O.step_seed(m, 300, 0), frame=300)
+    mx = float(np.abs(nxt.astype(np.float64) - want).max())
+    print(f"multikernel frame 300 from the GPU state vs FP64 oracle: max-abs {mx:.2e}")
+    assert mx <= 1e-5
+
+
 def test_scene_frame_rules(ctx):
     """per-frame parameters, freeze, reseed and visibility (SceneRunner.cpp:146-215)"""
     import lenia
