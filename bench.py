@@ -465,6 +465,7 @@ def main() -> int:
     ap.add_argument("--no-direct", action="store_true", help="skip the 1280x720 direct-forms leg")
     ap.add_argument("--no-matrix", action="store_true", help="skip the other configurations")
     ap.add_argument("--no-lifebench", action="store_true", help="skip the LifeBench scenes")
+    ap.add_argument("--no-live", action="store_true", help="skip the live-field rate")
     ap.add_argument("--tiles", default="", metavar="ROWT,ROWP,COLT,COLC",
                     help="fixed launch shapes of the headline world instead of autotuning "
                          "(row threads, row pairs per tile, column threads, columns per tile)")
@@ -521,7 +522,7 @@ def main() -> int:
         passes = w.pass_times(PASS_SAMPLES)
         tiles = w.resolved()
         A = w.read()
-        live = live_field_bench(ctx, args.steps)
+        live = None if args.no_live else live_field_bench(ctx, args.steps)
         batched = None if args.no_sweep else sweep_bench(ctx, args.sweep_steps,
                                                           plain_steps=args.plain_steps)
         adjoint = None if args.no_adjoint else adjoint_bench(ctx)
@@ -565,9 +566,8 @@ def main() -> int:
                      "step_achieved_gbs": round(step_gbs, 1),
                      "step_bytes_per_cell": round(fft_bytes_per_cell(W, K)["step"], 3)},
         "state_mean": float(A.mean()),
-        "live_field": live,
     }
-    for key, val in (("batched", batched), ("adjoint", adjoint), ("direct_forms", direct),
+    for key, val in (("live_field", live), ("batched", batched), ("adjoint", adjoint), ("direct_forms", direct),
                      ("matrix", matrix), ("lifebench", lifebench)):
         if val is not None:
             out[key] = val
