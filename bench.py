@@ -147,7 +147,25 @@ def live_field_bench(ctx, steps: int) -> dict:
             "live_cell_frac": round(live_cells(A) / A.size, 4), "state_std": round(float(A.std()), 4)}
 
 
-def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int = 64) -> dict:
+def tune_sweep_plans(ctx) -> dict:
+    """radix plans for the sweep's side lengths, each tuned on a square world of
+    that side with two R=8 ring kernels (lenia_plan_autotune); the sweep's
+    worlds are created afterwards and take them from the plan cache"""
+    import lenia
+    from lenia_cases import FFT, INIT_PATCH, SWEEP_SIDES, ring
+
+    out = {}
+    for n in SWEEP_SIDES:
+        d = dict(width=n, height=n, convMode=FFT, growthMode=0, dt=0.1, radius=8,
+                 kernels=[ring(8, [1.0], 0.2, 0.03, 0.5), ring(8, [1.0, 0.5], 0.3, 0.05, 0.5)],
+                 initMode=INIT_PATCH, initRect=(0, 0, 0, 0))
+        pw, ph, _ = lenia.plan_autotune(ctx, d, 6)
+        out[str(n)] = {"rows": hex(pw), "cols": hex(ph)}
+    return out
+
+
+def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int = 64,
+                tune_plans: bool = True) -> dict:
     """256 uneven worlds stepped in one batch (graph-replayed) vs the same worlds
     stepped one world per launch sequence, one after another (the PresetLab
     loop, Apps/LifePresetLab/main.cpp:718-806).  The plain loop replays each
@@ -162,6 +180,7 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
         raise SystemExit(f"bench.py: sweep steps ({steps}) must be a multiple of {graph_steps}")
     if plain_steps % 32:
         raise SystemExit("bench.py: --plain-steps must be a multiple of 32 (graph replay)")
+    plans = tune_sweep_plans(ctx) if tune_plans else None
     descs = sweep_worlds(256)
     worlds = [lenia.World(ctx, d) for d in descs]
     try:
@@ -204,7 +223,8 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
                 "plain": plain,
                 "speedup_vs_plain": plain["speedup_batched_vs_plain_wall"] if plain else None,
                 "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5)}
+                "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5),
+                "radix_plans": plans}
     finally:
         for w in worlds:
             w.close()
@@ -295,7 +315,7 @@ def direct_forms_bench(ctx, steps: int = 200) -> dict:
     return out
 
 
-def matrix_bench(ctx, cpu_s: float) -> dict:
+def matrix_bench(ctx, cpu_s: float, tune_plans: bool = True) -> dict:
     """The configurations other than the headline, each at its stated length:
     512^2 direct FP32 R=13 (200 steps), and the headline kernels' FFT form at
     1920x1080 and 4096^2 (500 steps, launch shapes autotuned).  Rates from HIP
@@ -336,6 +356,8 @@ def matrix_bench(ctx, cpu_s: float) -> dict:
     for W, H in ((1920, 1080), (4096, 4096)):
         d = config1(W, H)
         K = len(d["kernels"])
+        if tune_plans:
+            lenia.plan_autotune(ctx, d, 6)  # radix plans of this width and height
         with lenia.World(ctx, d) as w:
             w.reset(7777)
             w.frame(0, 1)
@@ -466,6 +488,8 @@ def main() -> int:
     ap.add_argument("--no-matrix", action="store_true", help="skip the other configurations")
     ap.add_argument("--no-lifebench", action="store_true", help="skip the LifeBench scenes")
     ap.add_argument("--no-live", action="store_true", help="skip the live-field rate")
+    ap.add_argument("--no-plan-tune", action="store_true",
+                    help="keep the built-in radix plans (no lenia_plan_autotune)")
     ap.add_argument("--tiles", default="", metavar="ROWT,ROWP,COLT,COLC",
                     help="fixed launch shapes of the headline world instead of autotuning "
                          "(row threads, row pairs per tile, column threads, columns per tile)")
@@ -489,9 +513,11 @@ def main() -> int:
     cpu_s = 0.0 if args.no_cpu else max(1.0, args.cpu_seconds / 4)
     if args.only:
         import lenia
-        legs = {"sweep": lambda c: sweep_bench(c, args.sweep_steps, plain_steps=args.plain_steps),
+        legs = {"sweep": lambda c: sweep_bench(c, args.sweep_steps, plain_steps=args.plain_steps,
+                                               tune_plans=not args.no_plan_tune),
                 "adjoint": adjoint_bench, "direct": direct_forms_bench,
-                "matrix": lambda c: matrix_bench(c, cpu_s), "lifebench": lifebench_bench,
+                "matrix": lambda c: matrix_bench(c, cpu_s, not args.no_plan_tune),
+                "lifebench": lifebench_bench,
                 "live": lambda c: live_field_bench(c, args.steps)}
         with lenia.Context(0) as ctx:
             print(json.dumps({args.only: legs[args.only](ctx)}))
@@ -506,7 +532,11 @@ def main() -> int:
         desc.update(rowThreads=rt, rowPairsPerTile=rp, colThreads=ct, colsPerTile=cc)
     W, H, K = desc["width"], desc["height"], len(desc["kernels"])
     cells = W * H
-    with lenia.Context(0) as ctx, lenia.World(ctx, desc) as w:
+    with lenia.Context(0) as ctx:
+        if not args.tiles and not args.no_plan_tune:
+            # radix plans of the headline world's width and height (plan cache)
+            lenia.plan_autotune(ctx, desc, 10)
+        w = lenia.World(ctx, desc)
         w.reset(7777)
         w.frame(0, 1)
         if not args.tiles:
@@ -522,12 +552,14 @@ def main() -> int:
         passes = w.pass_times(PASS_SAMPLES)
         tiles = w.resolved()
         A = w.read()
+        w.close()
         live = None if args.no_live else live_field_bench(ctx, args.steps)
         batched = None if args.no_sweep else sweep_bench(ctx, args.sweep_steps,
-                                                          plain_steps=args.plain_steps)
+                                                          plain_steps=args.plain_steps,
+                                                          tune_plans=not args.no_plan_tune)
         adjoint = None if args.no_adjoint else adjoint_bench(ctx)
         direct = None if args.no_direct else direct_forms_bench(ctx)
-        matrix = None if args.no_matrix else matrix_bench(ctx, cpu_s)
+        matrix = None if args.no_matrix else matrix_bench(ctx, cpu_s, not args.no_plan_tune)
         lifebench = None if args.no_lifebench else lifebench_bench(ctx)
     sec = gpu_ms / 1e3
     value = cells * args.steps / sec / 1e6
@@ -553,7 +585,8 @@ def main() -> int:
                    "fftPlanW": tiles["fftPlanW"], "fftPlanH": tiles["fftPlanH"],
                    "rowPairsPerTile": tiles["rowPairsPerTile"],
                    "colsPerTile": tiles["colsPerTile"], "rowThreads": tiles["rowThreads"],
-                   "colThreads": tiles["colThreads"], "autotuned": not args.tiles},
+                   "colThreads": tiles["colThreads"], "autotuned": not args.tiles,
+                   "radix_plans_tuned": not (args.tiles or args.no_plan_tune)},
         "passes": roof,
         "pass_samples": PASS_SAMPLES,
         "roofline": {"bound": "hbm", "kernel": roof[dom]["kernel"],

@@ -180,6 +180,32 @@ int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out);
  * step.  Direct forms: nothing to tune, *msPerStep = 0. */
 int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msPerStep);
 
+/* ---- radix plans (no counterpart in the reference, whose FFT is radix 2
+ * only; for callers that want tuned plans shared by single worlds and
+ * batches).  The plan of a transform length decides the arithmetic of every
+ * FFT-form step, so it belongs to the (axis, length) pair, never to a launch
+ * shape or a batch: a world created with fftPlanW / fftPlanH = 0 takes the
+ * plan cached for its width (axis 0, row transforms) and height (axis 1,
+ * column transforms), else the built-in default, and keeps it for life.
+ * Worlds created under the same cache therefore give the same bits alone and
+ * batched.  The cache is process-wide and thread-safe. */
+/* times every compiled radix plan of desc's width (row pass) and then of its
+ * height (column pass) on the FFT form of desc, launch shapes tuned per plan,
+ * nSteps steps each; records the fastest pair in the cache and returns it
+ * (packed as fftPlanW / fftPlanH; any output may be NULL) with its kernel ms
+ * per step.  Worlds that already exist keep their plans. */
+int lenia_plan_autotune(lenia_ctx *ctx, const lenia_world_desc *desc, uint32_t nSteps,
+                        uint32_t *planW, uint32_t *planH, double *msPerStep);
+/* one entry: axis 0 rows, 1 columns; plan 0 removes it; a plan that is not a
+ * radix sequence of length n is rejected */
+int lenia_plan_cache_set(uint32_t axis, uint32_t n, uint32_t plan);
+int lenia_plan_cache_get(uint32_t axis, uint32_t n, uint32_t *plan); /* 0: no entry */
+void lenia_plan_cache_clear(void);
+/* the cache as a text file, one "rows|cols <length> <plan hex>" line per entry
+ * ('#' starts a comment line); loading is all or nothing and adds to the cache */
+int lenia_plan_cache_save(const char *path);
+int lenia_plan_cache_load(const char *path);
+
 /* ---- allocation-free stepping (no counterpart in the reference): for callers
  * that cannot allocate device memory -- code under stream capture, serving
  * loops, harnesses that forbid hipMalloc.  Steps the caller's device field

@@ -374,9 +374,39 @@ __device__ __forceinline__ void fft_fixed(float2 *buf, int nt, const float2 *tw,
 }
 
 // fft_lds through the compile-time kernel of the plan when it has one
+// the other radix orders of the benchmark lengths (radix-plan tuning), kept
+// out of the default plans' code path
+template <bool INV, int MAXV, int T>
+__device__ __forceinline__ void fft_run_alt(float2 *buf, uint32_t code, int nt,
+                                                      const float2 *tw, int tid) {
+    switch (code) {
+        case kPlan64b: fft_fixed<INV, MAXV, T, 64, 1, 0, 16, 4>(buf, nt, tw, tid); break;
+        case kPlan64c: fft_fixed<INV, MAXV, T, 64, 1, 0, 4, 16>(buf, nt, tw, tid); break;
+        case kPlan128b: fft_fixed<INV, MAXV, T, 128, 1, 0, 8, 16>(buf, nt, tw, tid); break;
+        case kPlan128c: fft_fixed<INV, MAXV, T, 128, 1, 0, 4, 4, 8>(buf, nt, tw, tid); break;
+        case kPlan256b: fft_fixed<INV, MAXV, T, 256, 1, 0, 8, 8, 4>(buf, nt, tw, tid); break;
+        case kPlan256c: fft_fixed<INV, MAXV, T, 256, 1, 0, 4, 8, 8>(buf, nt, tw, tid); break;
+        case kPlan512b: fft_fixed<INV, MAXV, T, 512, 1, 0, 16, 8, 4>(buf, nt, tw, tid); break;
+        case kPlan512c: fft_fixed<INV, MAXV, T, 512, 1, 0, 4, 8, 16>(buf, nt, tw, tid); break;
+        case kPlan1024b: fft_fixed<INV, MAXV, T, 1024, 1, 0, 8, 8, 16>(buf, nt, tw, tid); break;
+        case kPlan1024c: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 16, 4>(buf, nt, tw, tid); break;
+        case kPlan1024d: fft_fixed<INV, MAXV, T, 1024, 1, 0, 4, 16, 16>(buf, nt, tw, tid); break;
+        case kPlan2048b: fft_fixed<INV, MAXV, T, 2048, 1, 0, 8, 16, 16>(buf, nt, tw, tid); break;
+        case kPlan4096b: fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, nt, tw, tid); break;
+        case kPlan96b: fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, nt, tw, tid); break;
+        case kPlan192b: fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, nt, tw, tid); break;
+        default: fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, nt, tw, tid); break;
+    }
+}
+
+// fft_lds through the compile-time kernel of the plan when it has one
 template <bool INV, int MAXV, int T>
 __device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
                                         int tid) {
+    if (__builtin_expect(plan.code >= kPlan64b, 0)) {
+        fft_run_alt<INV, MAXV, T>(buf, plan.code, nt, tw, tid);
+        return;
+    }
     switch (plan.code) {
         case kPlan64: fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, nt, tw, tid); break;
         case kPlan128: fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, nt, tw, tid); break;
@@ -391,6 +421,7 @@ __device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt
         case kPlan1080: fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, nt, tw, tid); break;
         case kPlan1280: fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, nt, tw, tid); break;
         case kPlan1920: fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, nt, tw, tid); break;
+        case kPlan4096: fft_fixed<INV, MAXV, T, 4096, 1, 0, 16, 16, 16>(buf, nt, tw, tid); break;
         default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid); break;
     }
 }

@@ -94,6 +94,9 @@ bool plan_make(int n, const int *radix, int ns, FftPlan &p);
 bool plan_default(int n, int maxR, FftPlan &p);
 bool plan_unpack(int n, uint32_t packed, FftPlan &p);
 uint32_t plan_pack(const FftPlan &p);
+int compiled_plans(int n, uint32_t *packed, int maxOut);
+// the plan cache's entry for (axis 0 rows / 1 columns, length n), 0 if none
+uint32_t plan_cache_lookup(uint32_t axis, uint32_t n);
 bool plan_fits(const FftPlan &p, int nt, int T, int maxv);
 size_t lds_rows(int W, int P, int K, bool kpar);
 size_t lds_cols(int H, int C, int K, bool kpar);

@@ -215,6 +215,30 @@ static uint32_t div_magic(int d) {
     return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
 }
 
+// The radix sequences with a compile-time kernel (fft_run): the default plan
+// of each common length, and for the benchmark lengths the other orders the
+// radix-plan autotuner times.
+namespace {
+struct Fixed {
+    int n, ns, r[5];
+    uint32_t code;
+};
+const Fixed kFixedPlans[] = {
+    {64, 2, {8, 8}, kPlan64},          {128, 2, {16, 8}, kPlan128},         {256, 2, {16, 16}, kPlan256},
+    {512, 3, {8, 8, 8}, kPlan512},     {1024, 3, {16, 8, 8}, kPlan1024},    {2048, 3, {16, 16, 8}, kPlan2048},
+    {96, 3, {8, 4, 3}, kPlan96},       {192, 3, {8, 8, 3}, kPlan192},       {384, 3, {16, 8, 3}, kPlan384},
+    {720, 4, {16, 3, 3, 5}, kPlan720}, {1080, 5, {8, 3, 3, 3, 5}, kPlan1080}, {1280, 3, {16, 16, 5}, kPlan1280},
+    {1920, 4, {16, 8, 3, 5}, kPlan1920}, {4096, 3, {16, 16, 16}, kPlan4096},
+    {64, 2, {16, 4}, kPlan64b},        {64, 2, {4, 16}, kPlan64c},
+    {128, 2, {8, 16}, kPlan128b},      {128, 3, {4, 4, 8}, kPlan128c},
+    {256, 3, {8, 8, 4}, kPlan256b},    {256, 3, {4, 8, 8}, kPlan256c},
+    {512, 3, {16, 8, 4}, kPlan512b},   {512, 3, {4, 8, 16}, kPlan512c},
+    {1024, 3, {8, 8, 16}, kPlan1024b}, {1024, 3, {16, 16, 4}, kPlan1024c},
+    {1024, 3, {4, 16, 16}, kPlan1024d}, {2048, 3, {8, 16, 16}, kPlan2048b},
+    {4096, 4, {8, 8, 8, 8}, kPlan4096b}, {96, 3, {4, 8, 3}, kPlan96b},
+    {192, 3, {16, 4, 3}, kPlan192b},   {384, 3, {8, 16, 3}, kPlan384b}};
+}  // namespace
+
 // fills the per-stage constants of a radix sequence; false if the product is
 // not n or a division magic is not exact on the index range it serves
 bool lenia::plan_make(int n, const int *radix, int ns, FftPlan &p) {
@@ -247,21 +271,21 @@ bool lenia::plan_make(int n, const int *radix, int ns, FftPlan &p) {
     if (prod != n) return false;
     p.n = n;
     p.nStages = ns;
-    // the radix sequences with a compile-time kernel
-    struct Fixed {
-        int n, ns, r[5];
-        uint32_t code;
-    };
-    static const Fixed fixed[] = {
-        {64, 2, {8, 8}, kPlan64},          {128, 2, {16, 8}, kPlan128},         {256, 2, {16, 16}, kPlan256},
-        {512, 3, {8, 8, 8}, kPlan512},     {1024, 3, {16, 8, 8}, kPlan1024},    {2048, 3, {16, 16, 8}, kPlan2048},
-        {96, 3, {8, 4, 3}, kPlan96},       {192, 3, {8, 8, 3}, kPlan192},       {384, 3, {16, 8, 3}, kPlan384},
-        {720, 4, {16, 3, 3, 5}, kPlan720}, {1080, 5, {8, 3, 3, 3, 5}, kPlan1080}, {1280, 3, {16, 16, 5}, kPlan1280},
-        {1920, 4, {16, 8, 3, 5}, kPlan1920}};
     p.code = kPlanGeneric;
-    for (const Fixed &f : fixed)
+    for (const Fixed &f : kFixedPlans)
         if (f.n == n && f.ns == ns && std::equal(radix, radix + ns, f.r)) p.code = f.code;
     return true;
+}
+
+// packed radix sequences of length n that have a compile-time kernel
+int lenia::compiled_plans(int n, uint32_t *packed, int maxOut) {
+    int m = 0;
+    for (const Fixed &f : kFixedPlans) {
+        if (f.n != n) continue;
+        FftPlan p;
+        if (m < maxOut && plan_make(n, f.r, f.ns, p)) packed[m++] = plan_pack(p);
+    }
+    return m;
 }
 
 // radix sequence for n: the power-of-two part in the fewest stages of radix
@@ -571,6 +595,9 @@ static bool choose_plan(int N, int K, uint32_t packed, FftPlan &p, bool rows) {
         return false;
     };
     if (packed) return plan_unpack(N, packed, p) && any_shape(p);
+    // a plan recorded for this length and axis (lenia_plan_autotune, plan cache)
+    const uint32_t cached = plan_cache_lookup(rows ? 0u : 1u, (uint32_t)N);
+    if (cached && plan_unpack(N, cached, p) && any_shape(p)) return true;
     for (int maxR = 16; maxR >= 2; maxR /= 2)
         if (plan_default(N, maxR, p) && any_shape(p)) return true;
     return false;

@@ -1,15 +1,28 @@
-// Empirical launch-shape tuning of one FFT-form world.
+// Empirical tuning of the FFT form: launch shapes and radix plans.
 //
-// The radix plan of a length is fixed (it decides the arithmetic, so it stays
-// put and results stay bit-identical); only the launch shape of each pass is
-// searched: transforms per workgroup (row pairs / columns), workgroup size and
-// whether the K inverse transforms of a tile run side by side.  Each
-// candidate is timed with per-kernel GPU timestamps over a few steps; the
+// lenia_world_autotune searches the launch shape of each pass of one world:
+// transforms per workgroup (row pairs / columns), workgroup size and whether
+// the K inverse transforms of a tile run side by side.  Shapes only decide
+// which threads run a butterfly, so results stay bit-identical; each
+// candidate is timed with per-kernel GPU timestamps over a few steps and the
 // world's state and step clock are restored afterwards.
+//
+// The radix plan of a transform length decides the arithmetic, so it is a
+// property of the (axis, length) pair held in a process-wide plan cache:
+// lenia_plan_autotune times every compiled plan of a world's row and column
+// lengths and records the fastest; worlds created afterwards without an
+// explicit plan take the cached one, so a world stepped alone and the same
+// world in a batch share plan and bits.  The cache is saved to and loaded
+// from a text file, one "rows|cols <length> <plan>" line per entry.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 #include "host_internal.h"
@@ -125,5 +138,163 @@ extern "C" int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msP
     if (rc2) return rc2;
     if (e != hipSuccess) return hip_fail(e, "autotune restore");
     if (msPerStep) *msPerStep = best[0] + best[1];
+    return LENIA_OK;
+}
+
+// ------------------------------------------------------------------ plan cache
+namespace {
+
+std::mutex g_planMu;
+std::map<std::pair<uint32_t, uint32_t>, uint32_t> g_plans;  // (axis, n) -> packed plan
+
+bool plan_valid(uint32_t n, uint32_t packed) {
+    FftPlan p;
+    return n >= 2 && n <= 4096 && packed != 0 && plan_unpack((int)n, packed, p);
+}
+
+}  // namespace
+
+uint32_t lenia::plan_cache_lookup(uint32_t axis, uint32_t n) {
+    std::lock_guard<std::mutex> lk(g_planMu);
+    auto it = g_plans.find({axis, n});
+    return it == g_plans.end() ? 0u : it->second;
+}
+
+extern "C" int lenia_plan_cache_set(uint32_t axis, uint32_t n, uint32_t plan) {
+    if (axis > 1) return fail(LENIA_ERR_INVALID_ARG, "plan cache: axis must be 0 (rows) or 1 (columns)");
+    if (plan != 0 && !plan_valid(n, plan))
+        return fail(LENIA_ERR_INVALID_ARG, "plan cache: 0x%x is not a radix plan of length %u", plan, n);
+    std::lock_guard<std::mutex> lk(g_planMu);
+    if (plan == 0)
+        g_plans.erase({axis, n});
+    else
+        g_plans[{axis, n}] = plan;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_plan_cache_get(uint32_t axis, uint32_t n, uint32_t *plan) {
+    if (!plan) return fail(LENIA_ERR_INVALID_ARG, "lenia_plan_cache_get: NULL argument");
+    if (axis > 1) return fail(LENIA_ERR_INVALID_ARG, "plan cache: axis must be 0 (rows) or 1 (columns)");
+    *plan = plan_cache_lookup(axis, n);
+    return LENIA_OK;
+}
+
+extern "C" void lenia_plan_cache_clear(void) {
+    std::lock_guard<std::mutex> lk(g_planMu);
+    g_plans.clear();
+}
+
+extern "C" int lenia_plan_cache_save(const char *path) {
+    if (!path) return fail(LENIA_ERR_INVALID_ARG, "lenia_plan_cache_save: NULL path");
+    std::vector<std::pair<std::pair<uint32_t, uint32_t>, uint32_t>> rows;
+    {
+        std::lock_guard<std::mutex> lk(g_planMu);
+        rows.assign(g_plans.begin(), g_plans.end());
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) return fail(LENIA_ERR_INVALID_ARG, "plan cache: cannot write %s (%s)", path, std::strerror(errno));
+    std::fprintf(f, "# lenia-mi355x radix plans: axis length plan (4 bits per stage, first stage lowest)\n");
+    for (const auto &r : rows)
+        std::fprintf(f, "%s %u 0x%x\n", r.first.first ? "cols" : "rows", r.first.second, r.second);
+    const bool ok = std::fclose(f) == 0;
+    return ok ? LENIA_OK : fail(LENIA_ERR_INVALID_ARG, "plan cache: write error on %s", path);
+}
+
+extern "C" int lenia_plan_cache_load(const char *path) {
+    if (!path) return fail(LENIA_ERR_INVALID_ARG, "lenia_plan_cache_load: NULL path");
+    FILE *f = std::fopen(path, "r");
+    if (!f) return fail(LENIA_ERR_INVALID_ARG, "plan cache: cannot read %s (%s)", path, std::strerror(errno));
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> got;
+    char line[256];
+    int lineNo = 0, rc = LENIA_OK;
+    while (rc == LENIA_OK && std::fgets(line, sizeof line, f)) {
+        ++lineNo;
+        char axis[16];
+        unsigned n = 0, plan = 0;
+        char extra[2];
+        const char *t = line;
+        while (*t == ' ' || *t == '\t') ++t;
+        if (*t == '#' || *t == '\n' || *t == '\r' || *t == 0) continue;
+        if (std::sscanf(t, "%15s %u %x %1s", axis, &n, &plan, extra) != 3 ||
+            (std::strcmp(axis, "rows") != 0 && std::strcmp(axis, "cols") != 0) || !plan_valid(n, plan))
+            rc = fail(LENIA_ERR_INVALID_ARG, "plan cache: %s line %d is not \"rows|cols <length> <plan>\" "
+                      "with a radix plan of that length", path, lineNo);
+        else
+            got[{std::strcmp(axis, "cols") == 0 ? 1u : 0u, n}] = plan;
+    }
+    std::fclose(f);
+    if (rc) return rc;  // all or nothing
+    std::lock_guard<std::mutex> lk(g_planMu);
+    for (const auto &e : got) g_plans[e.first] = e.second;
+    return LENIA_OK;
+}
+
+// ------------------------------------------------------------------ plan tuning
+namespace {
+
+// kernel ms per step of desc with the given plans and tuned launch shapes;
+// false when no shape serves the plans
+bool time_plans(lenia_ctx *ctx, lenia_world_desc d, uint32_t planW, uint32_t planH, uint32_t nSteps,
+                double &ms) {
+    d.fftPlanW = planW;
+    d.fftPlanH = planH;
+    d.rowPairsPerTile = d.colsPerTile = d.rowThreads = d.colThreads = 0;
+    lenia_world *w = nullptr;
+    if (lenia_world_create(ctx, &d, &w) != LENIA_OK) return false;
+    int rc = lenia_world_reset(w, 1u);
+    if (!rc) rc = lenia_world_frame(w, 0, 1);
+    if (!rc) rc = lenia_world_autotune(w, nSteps, &ms);
+    lenia_world_destroy(w);
+    return rc == LENIA_OK;
+}
+
+// the compiled plans of length n, or the built-in default when there is none
+std::vector<uint32_t> plan_candidates(int n) {
+    uint32_t c[16];
+    const int m = compiled_plans(n, c, 16);
+    std::vector<uint32_t> out(c, c + m);
+    if (out.empty()) {
+        FftPlan p;
+        if (plan_default(n, 16, p)) out.push_back(plan_pack(p));
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" int lenia_plan_autotune(lenia_ctx *ctx, const lenia_world_desc *desc, uint32_t nSteps,
+                                   uint32_t *planW, uint32_t *planH, double *msPerStep) {
+    if (!ctx || !desc) return fail(LENIA_ERR_INVALID_ARG, "lenia_plan_autotune: NULL argument");
+    if (desc->convMode != LENIA_CONV_FFT)
+        return fail(LENIA_ERR_UNSUPPORTED, "lenia_plan_autotune: FFT form only (convMode %u)", desc->convMode);
+    const int W = (int)desc->width, H = (int)desc->height;
+    nSteps = std::max(nSteps, 4u);
+    // rows first, with the column plan the cache or the default gives
+    const uint32_t colStart = plan_cache_lookup(1u, (uint32_t)H);
+    double bestMs = 1e30;
+    uint32_t bestW = 0, bestH = colStart;
+    for (uint32_t pw : plan_candidates(W)) {
+        double ms;
+        if (time_plans(ctx, *desc, pw, colStart, nSteps, ms) && ms < bestMs) {
+            bestMs = ms;
+            bestW = pw;
+        }
+    }
+    if (!bestW) return fail(LENIA_ERR_UNSUPPORTED, "lenia_plan_autotune: no radix plan of width %d runs", W);
+    bestMs = 1e30;
+    for (uint32_t ph : plan_candidates(H)) {
+        double ms;
+        if (time_plans(ctx, *desc, bestW, ph, nSteps, ms) && ms < bestMs) {
+            bestMs = ms;
+            bestH = ph;
+        }
+    }
+    if (!bestH) return fail(LENIA_ERR_UNSUPPORTED, "lenia_plan_autotune: no radix plan of height %d runs", H);
+    int rc = lenia_plan_cache_set(0u, (uint32_t)W, bestW);
+    if (!rc) rc = lenia_plan_cache_set(1u, (uint32_t)H, bestH);
+    if (rc) return rc;
+    if (planW) *planW = bestW;
+    if (planH) *planH = bestH;
+    if (msPerStep) *msPerStep = bestMs;
     return LENIA_OK;
 }

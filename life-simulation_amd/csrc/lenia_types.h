@@ -81,6 +81,25 @@ enum : uint32_t {
     kPlan1080 = 11, // 8 3 3 3 5
     kPlan1280 = 12, // 16 16 5
     kPlan1920 = 13, // 16 8 3 5
+    kPlan4096 = 14, // 16 16 16
+    // other radix orders of the benchmark lengths: candidates of the radix-plan
+    // autotuner (lenia_plan_autotune)
+    kPlan64b = 15,    // 16 4
+    kPlan64c = 16,    // 4 16
+    kPlan128b = 17,   // 8 16
+    kPlan128c = 18,   // 4 4 8
+    kPlan256b = 19,   // 8 8 4
+    kPlan256c = 20,   // 4 8 8
+    kPlan512b = 21,   // 16 8 4
+    kPlan512c = 22,   // 4 8 16
+    kPlan1024b = 23,  // 8 8 16
+    kPlan1024c = 24,  // 16 16 4
+    kPlan1024d = 25,  // 4 16 16
+    kPlan2048b = 26,  // 8 16 16
+    kPlan4096b = 27,  // 8 8 8 8
+    kPlan96b = 28,    // 4 8 3
+    kPlan192b = 29,   // 16 4 3
+    kPlan384b = 30,   // 8 16 3
 };
 
 struct FftPlan {

@@ -16,6 +16,8 @@ from ._abi import (CONV_DIRECT, CONV_DIRECT_BF16, CONV_DIRECT_BF16X3, CONV_FFT, 
                    SHAPE_SHELL, LeniaError, check)
 
 __all__ = ["Context", "World", "Batch", "Scene", "Sweep", "LeniaError", "describe_scene",
+           "plan_autotune", "plan_cache_set", "plan_cache_get", "plan_cache_clear",
+           "plan_cache_save", "plan_cache_load",
            "colormap_defaults", "colormap_field", "CONV_DIRECT", "CONV_FFT", "CONV_DIRECT_BF16",
            "CONV_DIRECT_BF16X3", "SHAPE_SHELL", "SHAPE_RING", "CORE_GAUSSIAN", "CORE_EXP",
            "INIT_PATCH", "INIT_NOISE", "INIT_NONE", "INIT_STAMP"]
@@ -242,6 +244,42 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+ROWS, COLS = 0, 1  # plan-cache axes
+
+
+def plan_autotune(ctx: Context, desc: dict, n_steps: int = 10):
+    """time every compiled radix plan of desc's width and height; the fastest
+    go into the plan cache (worlds created afterwards use them).  Returns
+    (planW, planH, kernel ms per step)."""
+    d = _abi.world_desc(desc)
+    pw, ph, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+    check(_abi.lib().lenia_plan_autotune(ctx.handle, C.byref(d), n_steps, C.byref(pw), C.byref(ph),
+                                         C.byref(ms)))
+    return pw.value, ph.value, ms.value
+
+
+def plan_cache_set(axis: int, n: int, plan: int) -> None:
+    check(_abi.lib().lenia_plan_cache_set(axis, n, plan))
+
+
+def plan_cache_get(axis: int, n: int) -> int:
+    v = C.c_uint32()
+    check(_abi.lib().lenia_plan_cache_get(axis, n, C.byref(v)))
+    return v.value
+
+
+def plan_cache_clear() -> None:
+    _abi.lib().lenia_plan_cache_clear()
+
+
+def plan_cache_save(path: str) -> None:
+    check(_abi.lib().lenia_plan_cache_save(str(path).encode()))
+
+
+def plan_cache_load(path: str) -> None:
+    check(_abi.lib().lenia_plan_cache_load(str(path).encode()))
 
 
 def describe_scene(text: str, module: int = 0):

@@ -43,7 +43,9 @@ EXPORTS = [
     "lenia_scene_step", "lenia_scene_set_param", "lenia_scene_get_info", "lenia_scene_world",
     "lenia_scene_layer", "lenia_sweep_create", "lenia_sweep_destroy", "lenia_sweep_count",
     "lenia_sweep_run", "lenia_sweep_activity", "lenia_sweep_world", "lenia_sweep_report",
-    "lenia_fft_step_scratch",
+    "lenia_fft_step_scratch", "lenia_plan_autotune", "lenia_plan_cache_set",
+    "lenia_plan_cache_get", "lenia_plan_cache_clear", "lenia_plan_cache_save",
+    "lenia_plan_cache_load",
 ]
 
 
@@ -206,6 +208,13 @@ def lib():
         "lenia_sweep_world": (i32, [vp, u32, C.POINTER(vp)]),
         "lenia_sweep_report": (i32, [vp, C.c_char_p, sz, C.POINTER(sz)]),
         "lenia_fft_step_scratch": (i32, [vp, C.POINTER(WorldDesc), u32, u32, vp, u32]),
+        "lenia_plan_autotune": (i32, [vp, C.POINTER(WorldDesc), u32, C.POINTER(u32), C.POINTER(u32),
+                                      C.POINTER(C.c_double)]),
+        "lenia_plan_cache_set": (i32, [u32, u32, u32]),
+        "lenia_plan_cache_get": (i32, [u32, u32, C.POINTER(u32)]),
+        "lenia_plan_cache_clear": (None, []),
+        "lenia_plan_cache_save": (i32, [C.c_char_p]),
+        "lenia_plan_cache_load": (i32, [C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -137,3 +137,39 @@ def test_c_caller_against_the_header(tmp_path):
     assert desc[14] == "24" and desc[16] == "0.350"
     kern = [ln.split() for ln in out.splitlines() if ln.startswith("kernel ")]
     assert [k[3] for k in kern] == ["24.000", "13.200", "7.200"]
+
+
+def test_plan_cache_set_get_save_load(tmp_path):
+    """the radix-plan cache is host state: entries are validated against their
+    length, removed with plan 0, saved as text and loaded all-or-nothing"""
+    import lenia
+    from lenia import _abi
+
+    lenia.plan_cache_clear()
+    assert lenia.plan_cache_get(lenia.ROWS, 1024) == 0
+    lenia.plan_cache_set(lenia.ROWS, 1024, 0x88F)   # 16, 8, 8
+    lenia.plan_cache_set(lenia.COLS, 512, 0x48F)    # 16, 8, 4
+    lenia.plan_cache_set(lenia.COLS, 96, 0x384)     # 4, 8, 3
+    assert lenia.plan_cache_get(lenia.ROWS, 1024) == 0x88F
+    assert lenia.plan_cache_get(lenia.COLS, 1024) == 0
+    for axis, n, plan in ((0, 1024, 0x88), (1, 96, 0x88), (2, 64, 0x88), (0, 100, 0x55)):
+        with pytest.raises(lenia.LeniaError) as ei:
+            lenia.plan_cache_set(axis, n, plan)
+        assert ei.value.code == _abi.ERR_INVALID_ARG
+    path = tmp_path / "plans.txt"
+    lenia.plan_cache_save(path)
+    text = path.read_text()
+    assert "rows 1024 0x88f" in text and "cols 512 0x48f" in text and "cols 96 0x384" in text
+    lenia.plan_cache_clear()
+    assert lenia.plan_cache_get(lenia.COLS, 512) == 0
+    lenia.plan_cache_load(path)
+    assert lenia.plan_cache_get(lenia.ROWS, 1024) == 0x88F
+    assert lenia.plan_cache_get(lenia.COLS, 96) == 0x384
+    lenia.plan_cache_set(lenia.ROWS, 1024, 0)
+    assert lenia.plan_cache_get(lenia.ROWS, 1024) == 0
+    bad = tmp_path / "bad.txt"
+    bad.write_text("# comment\nrows 256 0xff\ncols 256 0x3\n")  # 3 does not divide 256
+    with pytest.raises(lenia.LeniaError):
+        lenia.plan_cache_load(bad)
+    assert lenia.plan_cache_get(lenia.ROWS, 256) == 0  # nothing of the bad file was taken
+    lenia.plan_cache_clear()

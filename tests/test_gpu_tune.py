@@ -131,3 +131,91 @@ def test_column_pass_lean_variant_same_bits(ctx):
             w.step(12)
             out[ct] = w.read().tobytes()
     assert out[64] == out[256]
+
+
+# every radix sequence with a compile-time kernel (packed: 4 bits per stage,
+# first stage lowest, F = 16); the default plans and the tuner's alternatives
+COMPILED_PLANS = {
+    64: [0x88, 0x4F, 0xF4], 128: [0x8F, 0xF8, 0x844], 256: [0xFF, 0x488, 0x884],
+    512: [0x888, 0x48F, 0xF84], 1024: [0x88F, 0xF88, 0x4FF, 0xFF4],
+    2048: [0x8FF, 0xFF8], 4096: [0xFFF, 0x8888], 96: [0x348, 0x384], 192: [0x388, 0x34F],
+    384: [0x38F, 0x3F8],
+}
+
+
+@pytest.mark.parametrize("n", sorted(COMPILED_PLANS))
+def test_compiled_plans_match_oracle(ctx, n):
+    """each compiled radix plan of length n, as the row transform (width n) and
+    as the column transform (height n), one step within 1e-5 of the FP64 oracle"""
+    import lenia
+    import oracle as O
+    from lenia_cases import FFT, INIT_NONE, rand_state, ring
+
+    ks = [ring(8, [1.0], 0.2, 0.03, 0.6), ring(6, [1.0, 0.5], 0.3, 0.05, 0.4)]
+    worst = 0.0
+    for plan in COMPILED_PLANS[n]:
+        for W, H, key in ((n, 64, "fftPlanW"), (64, n, "fftPlanH")):
+            d = dict(width=W, height=H, convMode=FFT, growthMode=0, dt=0.1, radius=8, kernels=ks,
+                     initMode=INIT_NONE, **{key: plan})
+            A = rand_state(W, H, seed=plan & 0xFFF)
+            with lenia.World(ctx, d) as w:
+                assert w.resolved()[key] == plan
+                w.write(A)
+                w.step(1)
+                got = w.read()
+            want = O.step(d, A.astype(np.float64), use_fft=True)
+            err = float(np.abs(got - want).max())
+            worst = max(worst, err)
+            assert err <= 1e-5, (n, hex(plan), key, err)
+    print(f"length {n}: {len(COMPILED_PLANS[n])} plans, worst one-step error {worst:.2e}")
+
+
+def test_plan_autotune_shared_by_worlds_and_batches(ctx):
+    """lenia_plan_autotune records the fastest compiled plans of a world's
+    width and height; worlds created afterwards take them, a batch of such
+    worlds equals the same worlds stepped alone bit for bit, and the step is
+    within 1e-5 of the FP64 oracle.  An explicit plan still wins."""
+    import lenia
+    import oracle as O
+    from lenia_cases import INIT_NONE, rand_state
+
+    lenia.plan_cache_clear()
+    try:
+        desc = config1(W=512, H=256)
+        pw, ph, ms = lenia.plan_autotune(ctx, desc, 6)
+        print(f"512x256 K=3: tuned plans rows 0x{pw:x} cols 0x{ph:x}, {ms * 1e3:.1f} us per step")
+        assert pw in COMPILED_PLANS[512] and ph in COMPILED_PLANS[256]
+        assert lenia.plan_cache_get(lenia.ROWS, 512) == pw
+        assert lenia.plan_cache_get(lenia.COLS, 256) == ph
+        descs = [dict(desc, initMode=INIT_NONE), dict(desc, initMode=INIT_NONE, dt=0.05)]
+        states = [rand_state(512, 256, seed=s) for s in (1, 2)]
+        alone = []
+        for d, A in zip(descs, states):
+            with lenia.World(ctx, d) as w:
+                assert (w.resolved()["fftPlanW"], w.resolved()["fftPlanH"]) == (pw, ph)
+                w.write(A)
+                w.step(9)
+                alone.append(w.read())
+        worlds = [lenia.World(ctx, d) for d in descs]
+        try:
+            for w, A in zip(worlds, states):
+                w.write(A)
+            with lenia.Batch(ctx, worlds) as b:
+                b.step(9)
+            for w, want in zip(worlds, alone):
+                assert w.read().tobytes() == want.tobytes()
+        finally:
+            for w in worlds:
+                w.close()
+        with lenia.World(ctx, descs[0]) as w:
+            w.write(states[0])
+            w.step(1)
+            err = float(np.abs(w.read() - O.step(descs[0], states[0].astype(np.float64),
+                                                 use_fft=True)).max())
+        print(f"tuned plans, one step vs FP64 oracle: max-abs {err:.2e}")
+        assert err <= 1e-5
+        other = [p for p in COMPILED_PLANS[512] if p != pw][0]
+        with lenia.World(ctx, dict(descs[0], fftPlanW=other)) as w:
+            assert w.resolved()["fftPlanW"] == other
+    finally:
+        lenia.plan_cache_clear()
