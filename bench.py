@@ -199,6 +199,16 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
             ctx.sync()
             batch_wall = time.perf_counter() - t0
             batch_ms = ctx.last_gpu_ms()
+            # the launch form autotune did not pick, timed the same way:
+            # resident grids with tile queues vs one workgroup per tile
+            tuned_grids = b.grids()
+            alt_grids = (0, 0) if tuned_grids != (0, 0) else (1, 1)
+            b.set_grids(*alt_grids)
+            b.step(graph_steps, graph_steps)
+            b.step(steps, graph_steps)
+            ctx.sync()
+            alt_ms = ctx.last_gpu_ms()
+            b.set_grids(*tuned_grids)
         batch_rate = cells * steps / (batch_ms * 1e-3) / 1e6
         batch_wall_rate = cells * steps / batch_wall / 1e6
         plain = None
@@ -224,6 +234,8 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
                 "speedup_vs_plain": plain["speedup_batched_vs_plain_wall"] if plain else None,
                 "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5),
+                "grids_per_cu": {"tuned": list(tuned_grids), "alternative": list(alt_grids),
+                                 "alternative_ms_per_step": round(alt_ms / steps, 5)},
                 "radix_plans": plans}
     finally:
         for w in worlds:

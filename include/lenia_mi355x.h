@@ -262,6 +262,15 @@ int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
  * pass for all worlds) picked by timing every allowed choice over nSteps steps;
  * states and clocks restored, results unchanged bit for bit */
 int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep);
+/* FFT-form launch grids (no counterpart in the reference, which steps one
+ * variant at a time): with perCU > 0 a pass launches min(tiles, perCU x CUs)
+ * resident workgroups that take the batch's tiles from a queue, else one
+ * workgroup per tile (0, the default).  lenia_batch_autotune times 0, 1, 2
+ * and 4 per CU and keeps the fastest.  Queues apply to passes with side-by-
+ * side kernel transforms (and, for rows, no noise / jitter / inject terms);
+ * other passes stay one workgroup per tile.  Results are identical either way. */
+int lenia_batch_set_grids(lenia_batch *b, uint32_t colPerCU, uint32_t rowPerCU);
+int lenia_batch_get_grids(const lenia_batch *b, uint32_t *colPerCU, uint32_t *rowPerCU);
 /* nFrames consecutive frames frame0, frame0+1, ... of every world, substeps
  * steps each, with the frame / substep clock of LeniaModule::encode
  * (Lenia.cpp:562-564) -- what one SceneRunner::step per frame does to each

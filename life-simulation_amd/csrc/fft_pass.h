@@ -54,6 +54,48 @@ __device__ __forceinline__ int tile_of(int ti, int nTiles) {
     return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
 }
 
+// ---------------------------------------------------------------- tile queue
+// Persistent launches (batches): a resident grid takes tiles from the
+// world-ordered tile list instead of one workgroup per tile.  The list is
+// split into 8 contiguous chunks, one per XCD (workgroup ids reach the XCDs
+// round robin), so consecutive tiles of one world -- which share cache lines
+// and the twiddle table -- run on one XCD; a workgroup whose chunk is empty
+// takes from the next ones.  q[x] counts the tiles taken from chunk x and
+// q[8] the workgroups that found every chunk empty; the last of those resets
+// all nine for the next launch.  No workgroup ever waits for another.
+__device__ __forceinline__ int queue_next(int *q, int nTiles) {
+    const int x0 = (int)(blockIdx.x & 7u);
+    const int base = nTiles >> 3, rem = nTiles & 7;
+    for (int k = 0; k < 8; ++k) {
+        const int x = (x0 + k) & 7;
+        const int len = base + (x < rem ? 1 : 0);
+        const int i = atomicAdd(&q[x], 1);
+        if (i < len) return x * base + min(x, rem) + i;
+    }
+    return nTiles;
+}
+
+__device__ __forceinline__ void queue_exit(int *q) {
+    if (atomicAdd(&q[8], 1) == (int)gridDim.x - 1)
+        for (int x = 0; x < 9; ++x) atomicExch(&q[x], 0);
+}
+
+// The queue's next tile for this workgroup (nTiles: none left) and the
+// thread id to use for it.  The tile index is read back through LDS and made
+// wave-uniform, so the world descriptor stays in scalar registers; the thread
+// id passes an empty asm each time, so the values computed from it (the LDS
+// addresses of every FFT stage) are recomputed per tile instead of being
+// hoisted out of the tile loop and held in registers across it.
+__device__ __forceinline__ int queue_tile(int *queue, int nTiles, int *sTile, int &tid) {
+    if (threadIdx.x == 0) *sTile = queue_next(queue, nTiles);
+    __syncthreads();
+    const int ti = __builtin_amdgcn_readfirstlane(*sTile);
+    int t;
+    asm volatile("" : "=v"(t) : "0"((int)threadIdx.x));
+    tid = t;
+    return ti;
+}
+
 // ---------------------------------------------------------------- row pass
 // Twiddle-table copy global -> LDS, split in two so its loads can be in
 // flight together with the other loads of a tile
@@ -157,400 +199,456 @@ __device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, c
 // EXTRAS: the noise / muJitter / inject terms are compiled in (with all three
 // amounts zero both variants compute the same bits).
 // LDS: twiddles [W] | KPAR: Z [K][nt][W] ; serial: Z [nt][W] (sums in registers)
-template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T)>
-__global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
-                                                const int2 *__restrict__ tiles, int nTiles,
-                                                uint32_t stepOff, int kSel,
-                                                float *__restrict__ out) {
-    extern __shared__ float2 smem[];
+template <int MODE, int T, bool EXTRAS, bool KPAR, int V>
+__device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds, int2 tile,
+                                          bool newWorld, uint32_t stepOff, int kSel,
+                                          float *__restrict__ out, float2 *smem, int tid) {
     constexpr int kV = V, kG = kV / 2 + 1;
-    const int tid = threadIdx.x;
-    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
-    if (ti < nTiles) do {
-        const int2 tile = tiles[tile_of(ti, nTiles)];
-        const FftWorldDev &wd = worlds[tile.x];
-        const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
-        const int rp0 = tile.y * P;
-        const int nt = min(P, H2 - rp0);
-        const uint32_t mP = wd.mP;
-        float2 *tw = smem;
-        float2 *Z = smem + W;
-        LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
-        const int nq = nt * W;
-        const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
-        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twW));
-        TwCopy<T> twc;
-        twc.issue(twg, W, tid);
+    // the descriptor through the restrict table pointer: its fields may be
+    // reloaded instead of held in registers across the tile
+    const FftWorldDev &wd = worlds[tile.x];
+    const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
+    const int rp0 = tile.y * P;
+    const int nt = min(P, H2 - rp0);
+    const uint32_t mP = wd.mP;
+    float2 *tw = smem;
+    float2 *Z = smem + W;
+    LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
+    const int nq = nt * W;
+    const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
+    const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twW));
+    TwCopy<T> twc;
+    if (newWorld) twc.issue(twg, W, tid);
 
-        if (MODE == 0) {
-            twc.store(tw, twg, W, tid);
+    if (MODE == 0) {
+        if (newWorld) twc.store(tw, twg, W, tid);
+        for (int q = tid; q < nq; q += T) {
+            int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+            int ia = 2 * p * W + x;
+            Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
+        }
+        __syncthreads();
+        forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+        __syncthreads();
+        return;
+    }
+    const GrowthParams &g = wd.g;
+    const GrowthRegs gr = growth_regs(g);
+    LENIA_GLOBAL float *const saveU = gp(wd.saveU);
+    LENIA_GLOBAL float *const saveA = gp(wd.saveA);
+    const StepClock ck = clock_at(wd.clock, stepOff);
+    const uint32_t seed = clock_seed(ck, g.moduleSeed);
+    const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
+    const size_t gStride = (size_t)Wh * H2;
+    float ar0[kV], ar1[kV];
+    if (MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < kV; ++i) {
+            int q = tid + i * T;
+            if (q < nq) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ia = 2 * p * W + x;
+                ar0[i] = Arow[ia];
+                ar1[i] = Arow[ia + W];
+            }
+        }
+    }
+    if (KPAR || MODE == 2) {
+        const int kBeg = MODE == 2 ? kSel : 0;
+        const int kn = MODE == 2 ? 1 : (int)g.nKernels;
+        float4 gv[kG];
+        int zi[kG], kxv[kG];
+        gather_issue<T, kG>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
+        if (newWorld) twc.store(tw, twg, W, tid);
+        gather_store<kG>(Z, gv, zi, kxv, W);
+        __syncthreads();
+        fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
+        if (MODE == 2) {
             for (int q = tid; q < nq; q += T) {
                 int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
                 int ia = 2 * p * W + x;
-                Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
+                float2 z = Z[sw(q)];
+                LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
+                o[ia] = z.x;
+                o[ia + W] = z.y;
             }
             __syncthreads();
-            forward_rows<T>(Z, wd, tw, rp0, nt, tid);
-            __syncthreads();
-            continue;
+            return;
         }
-        const GrowthParams &g = wd.g;
-        const GrowthRegs gr = growth_regs(g);
-        LENIA_GLOBAL float *const saveU = gp(wd.saveU);
-        LENIA_GLOBAL float *const saveA = gp(wd.saveA);
-        const StepClock ck = clock_at(wd.clock, stepOff);
-        const uint32_t seed = clock_seed(ck, g.moduleSeed);
-        const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
-        const size_t gStride = (size_t)Wh * H2;
-        float ar0[kV], ar1[kV];
-        if (MODE == 1) {
 #pragma unroll
-            for (int i = 0; i < kV; ++i) {
-                int q = tid + i * T;
-                if (q < nq) {
-                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
-                    int ia = 2 * p * W + x;
-                    ar0[i] = Arow[ia];
-                    ar1[i] = Arow[ia + W];
+        for (int i = 0; i < kV; ++i) {
+            int q = tid + i * T;
+            if (q < nq) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ya = 2 * (rp0 + p);
+                float ja = 0.0f, jb = 0.0f;
+                if (EXTRAS) {
+                    ja = cell_jitter(g, x, ya, seed, ck.frame);
+                    jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
+                }
+                float2 z[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k < kn) z[k] = Z[sw(k * nq + q)];
+                float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k < kn) {
+                        sa += growth_term_r(gr, k, z[k].x, ja);
+                        sb += growth_term_r(gr, k, z[k].y, jb);
+                        if (saveU) {
+                            LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
+                            su[0] = z[k].x;
+                            su[W] = z[k].y;
+                        }
+                    }
+                }
+                if (saveA) {
+                    LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
+                    sa0[0] = ar0[i];
+                    sa0[W] = ar1[i];
+                }
+                if (EXTRAS) {
+                    ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
+                    ar1[i] = growth_finish(g, ar1[i], sb, x, ya + 1, W, H, seed, ck.frame);
+                } else {
+                    ar0[i] = growth_plain_r(gr, ar0[i], sa);
+                    ar1[i] = growth_plain_r(gr, ar1[i], sb);
                 }
             }
         }
-        if (KPAR || MODE == 2) {
-            const int kBeg = MODE == 2 ? kSel : 0;
-            const int kn = MODE == 2 ? 1 : (int)g.nKernels;
-            float4 gv[kG];
-            int zi[kG], kxv[kG];
-            gather_issue<T, kG>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
-            twc.store(tw, twg, W, tid);
-            gather_store<kG>(Z, gv, zi, kxv, W);
-            __syncthreads();
-            fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
-            if (MODE == 2) {
-                for (int q = tid; q < nq; q += T) {
-                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
-                    int ia = 2 * p * W + x;
-                    float2 z = Z[sw(q)];
-                    LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
-                    o[ia] = z.x;
-                    o[ia + W] = z.y;
-                }
-                __syncthreads();
-                continue;
+        // stores after all LDS reads (no read waits behind a store)
+#pragma unroll
+        for (int i = 0; i < kV; ++i) {
+            int q = tid + i * T;
+            if (q < nq) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ia = 2 * p * W + x;
+                Arow[ia] = ar0[i];
+                Arow[ia + W] = ar1[i];
+                Z[sw(q)] = make_float2(ar0[i], ar1[i]);
             }
+        }
+    } else {
+        float sa[kV], sb[kV];
+#pragma unroll
+        for (int i = 0; i < kV; ++i) sa[i] = sb[i] = 0.0f;
+        float4 gv[kG];
+        int zi[kG], kxv[kG];
+        gather_issue<T, kG>(gv, zi, kxv, G4, gStride, 0, 1, H2, nt, P, mP, nm, W, tid);
+        if (newWorld) twc.store(tw, twg, W, tid);
+        for (int k = 0; k < (int)g.nKernels; ++k) {
+            gather_store<kG>(Z, gv, zi, kxv, W);
+            // the next kernel's spectrum loads fly during this transform
+            if (k + 1 < (int)g.nKernels)
+                gather_issue<T, kG>(gv, zi, kxv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, W, tid);
+            __syncthreads();
+            // table-driven transform inside the runtime kernel loop: the
+            // compile-time one would keep its addresses live across it
+            fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
 #pragma unroll
             for (int i = 0; i < kV; ++i) {
                 int q = tid + i * T;
                 if (q < nq) {
                     int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
                     int ya = 2 * (rp0 + p);
+                    float2 z = Z[sw(q)];
                     float ja = 0.0f, jb = 0.0f;
                     if (EXTRAS) {
                         ja = cell_jitter(g, x, ya, seed, ck.frame);
                         jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
                     }
-                    float2 z[4];
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        if (k < kn) z[k] = Z[sw(k * nq + q)];
-                    float sa = 0.0f, sb = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (k < kn) {
-                            sa += growth_term_r(gr, k, z[k].x, ja);
-                            sb += growth_term_r(gr, k, z[k].y, jb);
-                            if (saveU) {
-                                LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
-                                su[0] = z[k].x;
-                                su[W] = z[k].y;
-                            }
-                        }
-                    }
-                    if (saveA) {
-                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
-                        sa0[0] = ar0[i];
-                        sa0[W] = ar1[i];
-                    }
-                    if (EXTRAS) {
-                        ar0[i] = growth_finish(g, ar0[i], sa, x, ya, W, H, seed, ck.frame);
-                        ar1[i] = growth_finish(g, ar1[i], sb, x, ya + 1, W, H, seed, ck.frame);
-                    } else {
-                        ar0[i] = growth_plain_r(gr, ar0[i], sa);
-                        ar1[i] = growth_plain_r(gr, ar1[i], sb);
+                    sa[i] += growth_term_r(gr, k, z.x, ja);
+                    sb[i] += growth_term_r(gr, k, z.y, jb);
+                    if (saveU) {
+                        LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
+                        su[0] = z.x;
+                        su[W] = z.y;
                     }
                 }
             }
-            // stores after all LDS reads (no read waits behind a store)
+            __syncthreads();
+        }
 #pragma unroll
-            for (int i = 0; i < kV; ++i) {
-                int q = tid + i * T;
-                if (q < nq) {
-                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
-                    int ia = 2 * p * W + x;
-                    Arow[ia] = ar0[i];
-                    Arow[ia + W] = ar1[i];
-                    Z[sw(q)] = make_float2(ar0[i], ar1[i]);
+        for (int i = 0; i < kV; ++i) {
+            int q = tid + i * T;
+            if (q < nq) {
+                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int ya = 2 * (rp0 + p);
+                float a2, b2;
+                if (saveA) {
+                    LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
+                    sa0[0] = ar0[i];
+                    sa0[W] = ar1[i];
                 }
-            }
-        } else {
-            float sa[kV], sb[kV];
-#pragma unroll
-            for (int i = 0; i < kV; ++i) sa[i] = sb[i] = 0.0f;
-            float4 gv[kG];
-            int zi[kG], kxv[kG];
-            gather_issue<T, kG>(gv, zi, kxv, G4, gStride, 0, 1, H2, nt, P, mP, nm, W, tid);
-            twc.store(tw, twg, W, tid);
-            for (int k = 0; k < (int)g.nKernels; ++k) {
-                gather_store<kG>(Z, gv, zi, kxv, W);
-                // the next kernel's spectrum loads fly during this transform
-                if (k + 1 < (int)g.nKernels)
-                    gather_issue<T, kG>(gv, zi, kxv, G4, gStride, k + 1, 1, H2, nt, P, mP, nm, W, tid);
-                __syncthreads();
-                // table-driven transform inside the runtime kernel loop: the
-                // compile-time one would keep its addresses live across it
-                fft_lds<true, kMaxV, T>(Z, wd.planW, nt, tw, tid);
-#pragma unroll
-                for (int i = 0; i < kV; ++i) {
-                    int q = tid + i * T;
-                    if (q < nq) {
-                        int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
-                        int ya = 2 * (rp0 + p);
-                        float2 z = Z[sw(q)];
-                        float ja = 0.0f, jb = 0.0f;
-                        if (EXTRAS) {
-                            ja = cell_jitter(g, x, ya, seed, ck.frame);
-                            jb = cell_jitter(g, x, ya + 1, seed, ck.frame);
-                        }
-                        sa[i] += growth_term_r(gr, k, z.x, ja);
-                        sb[i] += growth_term_r(gr, k, z.y, jb);
-                        if (saveU) {
-                            LENIA_GLOBAL float *su = saveU + (size_t)k * W * H + (size_t)ya * W + x;
-                            su[0] = z.x;
-                            su[W] = z.y;
-                        }
-                    }
+                if (EXTRAS) {
+                    a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
+                    b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
+                } else {
+                    a2 = growth_plain_r(gr, ar0[i], sa[i]);
+                    b2 = growth_plain_r(gr, ar1[i], sb[i]);
                 }
-                __syncthreads();
-            }
-#pragma unroll
-            for (int i = 0; i < kV; ++i) {
-                int q = tid + i * T;
-                if (q < nq) {
-                    int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
-                    int ya = 2 * (rp0 + p);
-                    float a2, b2;
-                    if (saveA) {
-                        LENIA_GLOBAL float *sa0 = saveA + (size_t)ya * W + x;
-                        sa0[0] = ar0[i];
-                        sa0[W] = ar1[i];
-                    }
-                    if (EXTRAS) {
-                        a2 = growth_finish(g, ar0[i], sa[i], x, ya, W, H, seed, ck.frame);
-                        b2 = growth_finish(g, ar1[i], sb[i], x, ya + 1, W, H, seed, ck.frame);
-                    } else {
-                        a2 = growth_plain_r(gr, ar0[i], sa[i]);
-                        b2 = growth_plain_r(gr, ar1[i], sb[i]);
-                    }
-                    int ia = 2 * p * W + x;
-                    Arow[ia] = a2;
-                    Arow[ia + W] = b2;
-                    Z[sw(q)] = make_float2(a2, b2);
-                }
+                int ia = 2 * p * W + x;
+                Arow[ia] = a2;
+                Arow[ia + W] = b2;
+                Z[sw(q)] = make_float2(a2, b2);
             }
         }
-        __syncthreads();
-        forward_rows<T>(Z, wd, tw, rp0, nt, tid);
-        __syncthreads();
-    } while (0);
+    }
+    __syncthreads();
+    forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+    __syncthreads();
+}
+
+// One tile per workgroup, or (QUEUE, mode 1 only) a resident grid taking
+// tiles from `queue`
+template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T), bool QUEUE = false>
+__global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles,
+                                                uint32_t stepOff, int kSel,
+                                                float *__restrict__ out, int *queue) {
+    extern __shared__ float2 smem[];
+    if constexpr (QUEUE) {
+        __shared__ int sTile;
+        int prevWorld = -1;
+        for (;;) {
+            int tid;
+            const int ti = queue_tile(queue, nTiles, &sTile, tid);
+            if (ti >= nTiles) break;
+            const int2 tile = tiles[ti];
+            // the twiddle table in LDS still holds this world's when the
+            // previous tile of this workgroup was of the same world
+            rows_tile<MODE, T, EXTRAS, KPAR, V>(worlds, tile, tile.x != prevWorld, stepOff, kSel,
+                                                out, smem, tid);
+            prevWorld = tile.x;
+        }
+        if (threadIdx.x == 0) queue_exit(queue);
+    } else if ((int)blockIdx.x < nTiles) {  // one tile per workgroup (grid = nTiles)
+        const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
+        rows_tile<MODE, T, EXTRAS, KPAR, V>(worlds, tile, true, stepOff, kSel, out, smem,
+                                            (int)threadIdx.x);
+    }
 }
 
 // ---------------------------------------------------------------- column pass
 // LDS: twiddles [H] | X [C][H] | KPAR: Y [K][C][H]
 // V: column values per thread held in registers (kMaxV; a tile of at most
 // kLeanCols per thread runs the variant sized for it)
-template <int T, bool KPAR, int V = kMaxV>
-__global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
-                                                const int2 *__restrict__ tiles, int nTiles) {
-    extern __shared__ float2 smem[];
-    const int tid = threadIdx.x;
-    const int ti = blockIdx.x;  // one tile per workgroup (grid = nTiles)
-    if (ti < nTiles) do {
-        const int2 tile = tiles[tile_of(ti, nTiles)];
-        const FftWorldDev &wd = worlds[tile.x];
-        const int H = wd.H, Wh = wd.Wh;
-        const int c0 = tile.y * wd.C;
-        const int nt = min(wd.C, Wh - c0);
-        const int ne = nt * H;
-        const int K = (int)wd.g.nKernels;
-        float2 *tw = smem;
-        float2 *X = smem + H;
-        const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twH));
-        TwCopy<T> twc;
-        twc.issue(twg, H, tid);
-        const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + (size_t)c0 * H;
-        const LENIA_GLOBAL float *S0 = gp(wd.S) + (size_t)c0 * H;
-        const size_t sStride = (size_t)Wh * H;
-        LENIA_GLOBAL float2 *G0 = gp(reinterpret_cast<float2 *>(wd.G)) + (size_t)c0 * H;
-        {
-            float2 fv[V];
+template <int T, bool KPAR, int V>
+__device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds, int2 tile,
+                                          bool newWorld, float2 *smem, int tid) {
+    const FftWorldDev &wd = worlds[tile.x];
+    const int H = wd.H, Wh = wd.Wh;
+    const int c0 = tile.y * wd.C;
+    const int nt = min(wd.C, Wh - c0);
+    const int ne = nt * H;
+    const int K = (int)wd.g.nKernels;
+    float2 *tw = smem;
+    float2 *X = smem + H;
+    const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twH));
+    TwCopy<T> twc;
+    if (newWorld) twc.issue(twg, H, tid);
+    const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + (size_t)c0 * H;
+    const LENIA_GLOBAL float *S0 = gp(wd.S) + (size_t)c0 * H;
+    const size_t sStride = (size_t)Wh * H;
+    LENIA_GLOBAL float2 *G0 = gp(reinterpret_cast<float2 *>(wd.G)) + (size_t)c0 * H;
+    {
+        float2 fv[V];
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < ne) fv[i] = gld2(F + e);
-            }
-            twc.store(tw, twg, H, tid);
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < ne) X[sw(e)] = fv[i];
-            }
+        for (int i = 0; i < V; ++i) {
+            int e = tid + i * T;
+            if (e < ne) fv[i] = gld2(F + e);
         }
-        if (KPAR) {
-            float2 *Y = X + lds_span(ne);
-            const int nk = K * ne;
-            float sr[V];
+        if (newWorld) twc.store(tw, twg, H, tid);
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < nk) {
-                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
-                    // (k, c) offsets fit 32 bits (K Wh H < 2^31): no 64-bit multiply
-                    sr[i] = S0[k * (int)sStride + c];
-                }
-            }
-            __syncthreads();
-            fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
-            if (wd.saveAhat) {
-                LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
-                for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
-            }
-            // all reads of the forward spectrum first, then the scaled copies
-            float2 xv[V];
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < nk) {
-                    int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
-                    xv[i] = X[sw(c)];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
-            }
-            __syncthreads();
-            fft_run<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
-            for (int e = tid; e < nk; e += T) {
-                int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
-                gst2(G0 + (k * (int)sStride + c), Y[sw(e)]);
-            }
-            __syncthreads();
-            continue;
+        for (int i = 0; i < V; ++i) {
+            int e = tid + i * T;
+            if (e < ne) X[sw(e)] = fv[i];
         }
+    }
+    if (KPAR) {
+        float2 *Y = X + lds_span(ne);
+        const int nk = K * ne;
         float sr[V];
 #pragma unroll
         for (int i = 0; i < V; ++i) {
             int e = tid + i * T;
-            if (e < ne) sr[i] = S0[e];
+            if (e < nk) {
+                int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                // (k, c) offsets fit 32 bits (K Wh H < 2^31): no 64-bit multiply
+                sr[i] = S0[k * (int)sStride + c];
+            }
         }
         __syncthreads();
         fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
-        float2 xs[V];
+        if (wd.saveAhat) {
+            LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+            for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);
+        }
+        // all reads of the forward spectrum first, then the scaled copies
+        float2 xv[V];
 #pragma unroll
         for (int i = 0; i < V; ++i) {
             int e = tid + i * T;
-            xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
+            if (e < nk) {
+                int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+                xv[i] = X[sw(c)];
+            }
         }
-        if (wd.saveAhat) {
-            LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            int e = tid + i * T;
+            if (e < nk) Y[sw(e)] = cscale(xv[i], sr[i]);
+        }
+        __syncthreads();
+        fft_run<true, kMaxV, T>(Y, wd.planH, K * nt, tw, tid);
+        for (int e = tid; e < nk; e += T) {
+            int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
+            gst2(G0 + (k * (int)sStride + c), Y[sw(e)]);
+        }
+        __syncthreads();
+        return;
+    }
+    float sr[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        int e = tid + i * T;
+        if (e < ne) sr[i] = S0[e];
+    }
+    __syncthreads();
+    fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+    float2 xs[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        int e = tid + i * T;
+        xs[i] = e < ne ? X[sw(e)] : make_float2(0.f, 0.f);
+    }
+    if (wd.saveAhat) {
+        LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            int e = tid + i * T;
+            if (e < ne) gst2(sv + e, xs[i]);
+        }
+    }
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            int e = tid + i * T;
+            if (e < ne) X[sw(e)] = cscale(xs[i], sr[i]);
+        }
+        if (k + 1 < K) {
+            const LENIA_GLOBAL float *Sn = S0 + (size_t)(k + 1) * sStride;
 #pragma unroll
             for (int i = 0; i < V; ++i) {
                 int e = tid + i * T;
-                if (e < ne) gst2(sv + e, xs[i]);
+                if (e < ne) sr[i] = Sn[e];
             }
         }
-        for (int k = 0; k < K; ++k) {
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-                int e = tid + i * T;
-                if (e < ne) X[sw(e)] = cscale(xs[i], sr[i]);
-            }
-            if (k + 1 < K) {
-                const LENIA_GLOBAL float *Sn = S0 + (size_t)(k + 1) * sStride;
-#pragma unroll
-                for (int i = 0; i < V; ++i) {
-                    int e = tid + i * T;
-                    if (e < ne) sr[i] = Sn[e];
-                }
-            }
-            __syncthreads();
-            fft_lds<true, kMaxV, T>(X, wd.planH, nt, tw, tid);
-            LENIA_GLOBAL float2 *G = G0 + (size_t)k * sStride;
-            for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
-            __syncthreads();
+        __syncthreads();
+        fft_lds<true, kMaxV, T>(X, wd.planH, nt, tw, tid);
+        LENIA_GLOBAL float2 *G = G0 + (size_t)k * sStride;
+        for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
+        __syncthreads();
+    }
+}
+
+// One tile per workgroup, or (QUEUE) a resident grid taking tiles from `queue`
+template <int T, bool KPAR, int V = kMaxV, bool QUEUE = false>
+__global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
+                                                const int2 *__restrict__ tiles, int nTiles,
+                                                int *queue) {
+    extern __shared__ float2 smem[];
+    if constexpr (QUEUE) {
+        __shared__ int sTile;
+        int prevWorld = -1;
+        for (;;) {
+            int tid;
+            const int ti = queue_tile(queue, nTiles, &sTile, tid);
+            if (ti >= nTiles) break;
+            const int2 tile = tiles[ti];
+            cols_tile<T, KPAR, V>(worlds, tile, tile.x != prevWorld, smem, tid);
+            prevWorld = tile.x;
         }
-    } while (0);
+        if (threadIdx.x == 0) queue_exit(queue);
+    } else if ((int)blockIdx.x < nTiles) {  // one tile per workgroup (grid = nTiles)
+        const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
+        cols_tile<T, KPAR, V>(worlds, tile, true, smem, (int)threadIdx.x);
+    }
 }
 
 // ---------------------------------------------------------------- launch templates
 template <int T, bool EX, bool KP, int V = row_values(T)>
 void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
                         int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                        hipStream_t s) {
+                        hipStream_t s, int *queue = nullptr) {
     if (mode == 0)
         launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, stepOff, kSel, out);
-    else if (mode == 1)
+                     nTiles, stepOff, kSel, out, queue);
+    else if (mode == 1 && queue) {
+        // tile queues: side-by-side transforms without the extra terms only
+        if constexpr (KP && !EX)
+            launch_timed(k_fft_rows<1, T, false, true, V, true>, dim3(grid), dim3(T), lds, s, worlds,
+                         tiles, nTiles, stepOff, kSel, out, queue);
+    } else if (mode == 1)
         launch_timed(k_fft_rows<1, T, EX, KP, V>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
-                     stepOff, kSel, out);
+                     stepOff, kSel, out, queue);
     else
         launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, stepOff, kSel, out);
+                     nTiles, stepOff, kSel, out, queue);
 }
 
 template <int T>
 void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
                           const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
-                          int kSel, float *out, hipStream_t s) {
+                          int kSel, float *out, hipStream_t s, int *queue) {
     if (extras) {
-        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
-        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
     } else {
-        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
-        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
     }
 }
 
 template <int T>
 void cols_dispatch(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,
-                   int nTiles, int grid, size_t lds, hipStream_t s) {
-    if (kpar && lean)
+                   int nTiles, int grid, size_t lds, hipStream_t s, int *queue) {
+    if (kpar && lean && queue)
+        launch_timed(k_fft_cols<T, true, kLeanCols, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, queue);
+    else if (kpar && queue)
+        launch_timed(k_fft_cols<T, true, kMaxV, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
+                     nTiles, queue);
+    else if (queue)
+        return;  // tile queues: side-by-side transforms only
+    else if (kpar && lean)
         launch_timed(k_fft_cols<T, true, kLeanCols>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles);
+                     nTiles, queue);
     else if (kpar)
-        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, queue);
     else
-        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
+        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, queue);
 }
 
 // per-workgroup-size entry points (k_fft_pass<T>.hip)
 #define LENIA_FFT_PASS_DECL(T)                                                                  \
     void fft_rows_pass_##T(int mode, bool extras, bool kpar, const FftWorldDev *worlds,         \
                            const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff, \
-                           int kSel, float *out, hipStream_t s);                                \
+                           int kSel, float *out, hipStream_t s, int *queue);                    \
     void fft_cols_pass_##T(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,   \
-                           int nTiles, int grid, size_t lds, hipStream_t s);
+                           int nTiles, int grid, size_t lds, hipStream_t s, int *queue);
 LENIA_FFT_PASS_DECL(64)
 LENIA_FFT_PASS_DECL(128)
 LENIA_FFT_PASS_DECL(256)
 LENIA_FFT_PASS_DECL(512)
 // 256- / 512-thread step passes holding kMidValues cells per thread (side-by-side transforms)
 void fft_rows_step_256_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t stepOff, hipStream_t s);
+                           int grid, size_t lds, uint32_t stepOff, hipStream_t s, int *queue);
 void fft_rows_step_512_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t stepOff, hipStream_t s);
+                           int grid, size_t lds, uint32_t stepOff, hipStream_t s, int *queue);
 void fft_rows_pass_1024(int mode, bool extras, bool lean, const FftWorldDev *worlds, const int2 *tiles,
                         int nTiles, int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                        hipStream_t s);
+                        hipStream_t s, int *queue);
 
 }  // namespace lenia

@@ -100,8 +100,9 @@ uint32_t plan_cache_lookup(uint32_t axis, uint32_t n);
 bool plan_fits(const FftPlan &p, int nt, int T, int maxv);
 size_t lds_rows(int W, int P, int K, bool kpar);
 size_t lds_cols(int H, int C, int K, bool kpar);
-// LDS a gfx950 workgroup can address (160 KB), the limit of FFT shapes
-constexpr size_t kFftLdsMax = 163840;
+// LDS a gfx950 workgroup can address (160 KB) less what the step kernels
+// declare statically (the tile-queue slot): the limit of FFT shapes
+constexpr size_t kFftLdsMax = 163840 - 256;
 bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
 bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
                 int &ntOut, int &TOut, bool &kparOut);

@@ -37,10 +37,11 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                            size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s,
-                           int tileCells = 0);
+                           int tileCells = 0, int *queue = nullptr);
 // tileCells: the largest (K) x nt x H over the launch's tiles, or 0
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
-                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells = 0);
+                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells = 0,
+                           int *queue = nullptr);
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s);
 hipError_t launch_stage_twiddles(const FftPlan &plan, const float *raw, float *out, hipStream_t s);
 hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double *Cw, float *S,

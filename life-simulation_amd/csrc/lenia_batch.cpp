@@ -39,6 +39,13 @@ struct lenia_batch {
     // launch shapes fixed by lenia_batch_autotune (0: the default choice)
     int tuneRowT = 0, tuneColT = 0;
     bool tuneRowKpar = true, tuneColKpar = true;
+    // resident grids of workgroups per CU for the column / row pass, which
+    // then take tiles from a queue (0: one workgroup per tile)
+    int colGridPerCU = 0, rowGridPerCU = 0;
+    // tile-queue counters: 9 for the column pass at 0, 9 for the row pass at
+    // kRowQueue; each launch leaves its nine at zero
+    int *queue = nullptr;
+    static constexpr int kRowQueue = 16;
 };
 
 static void batch_release_tables(lenia_batch *b) {
@@ -47,8 +54,10 @@ static void batch_release_tables(lenia_batch *b) {
     dev_free(b->ctx, b->devWorlds);
     dev_free(b->ctx, b->rowTiles);
     dev_free(b->ctx, b->colTiles);
+    dev_free(b->ctx, b->queue);
     b->devWorlds = nullptr;
     b->rowTiles = b->colTiles = nullptr;
+    b->queue = nullptr;
     b->ldsRows = b->ldsCols = 0;
 }
 
@@ -143,9 +152,13 @@ static int batch_setup_fft(lenia_batch *b) {
     int rc = dev_alloc(ctx, &b->devWorlds, sizeof(FftWorldDev) * n);
     if (!rc) rc = dev_alloc(ctx, (void **)&b->rowTiles, sizeof(int2) * rt.size());
     if (!rc) rc = dev_alloc(ctx, (void **)&b->colTiles, sizeof(int2) * ct.size());
+    if (!rc) rc = dev_alloc(ctx, (void **)&b->queue, sizeof(int) * 2 * lenia_batch::kRowQueue);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(FftWorldDev) * n,
-                                  hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = launch_fill(reinterpret_cast<float *>(b->queue), 2 * lenia_batch::kRowQueue, 0.0f,
+                               ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(FftWorldDev) * n,
+                           hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->rowTiles, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice,
                            ctx->stream);
@@ -229,14 +242,20 @@ static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint3
     const int n = (int)b->worlds.size();
     hipError_t e;
     if (b->isFft) {
+        // a resident grid smaller than the tile list takes tiles from a queue
+        const int cg = b->colKpar ? std::min(b->nColTiles, b->colGridPerCU * ctx->numCUs) : 0;
+        const int rg = b->rowKpar && !b->extras ? std::min(b->nRowTiles, b->rowGridPerCU * ctx->numCUs) : 0;
+        const bool cq = cg > 0 && cg < b->nColTiles, rq = rg > 0 && rg < b->nRowTiles;
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds, b->colTiles,
-                            b->nColTiles, b->nColTiles, b->ldsCols, ctx->stream, b->colCells);
+                            b->nColTiles, cq ? cg : b->nColTiles, b->ldsCols, ctx->stream,
+                            b->colCells, cq ? b->queue : nullptr);
         if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
             e = launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, (const FftWorldDev *)b->devWorlds,
-                                b->rowTiles, b->nRowTiles, b->nRowTiles, b->ldsRows, off, 0,
-                                nullptr, ctx->stream, b->rowCells);
+                                b->rowTiles, b->nRowTiles, rq ? rg : b->nRowTiles, b->ldsRows, off, 0,
+                                nullptr, ctx->stream, b->rowCells,
+                                rq ? b->queue + lenia_batch::kRowQueue : nullptr);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
@@ -338,8 +357,10 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     double best[2] = {1e30, 1e30};
     int bestRowT = 0, bestColT = 0;
     bool bestRowKp = true, bestColKp = true;
-    // columns keep their default while the row shapes are timed, and back
+    // columns keep their default while the row shapes are timed, and back;
+    // both passes one workgroup per tile until the grids are timed
     b->tuneColT = 0;
+    b->colGridPerCU = b->rowGridPerCU = 0;
     for (int kp = 1; kp >= 0 && !rc; --kp)
         for (int T = 64; T <= 512 && !rc; T *= 2) {
             b->tuneRowT = T;
@@ -368,6 +389,25 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
         }
     b->tuneColT = bestColT;
     b->tuneColKpar = bestColKp;
+    // then resident grids with tile queues against one workgroup per tile
+    // (a row: its best grid so far, ms[1]; columns: ms[0])
+    b->colGridPerCU = b->rowGridPerCU = 0;
+    int bestColGrid = 0, bestRowGrid = 0;
+    for (int gpc = 1; gpc <= 4 && !rc && bestRowT && bestColT; gpc *= 2) {
+        b->colGridPerCU = b->rowGridPerCU = gpc;
+        double ms[2];
+        if (time_now(ms) != LENIA_OK) continue;
+        if (ms[0] < best[0]) {
+            best[0] = ms[0];
+            bestColGrid = gpc;
+        }
+        if (ms[1] < best[1]) {
+            best[1] = ms[1];
+            bestRowGrid = gpc;
+        }
+    }
+    b->colGridPerCU = bestColGrid;
+    b->rowGridPerCU = bestRowGrid;
     const int rc2 = batch_setup(b);
     // restore
     off = 0;
@@ -390,6 +430,29 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     if (rc2) return rc2;
     if (e != hipSuccess) return hip_fail(e, "batch autotune restore");
     if (msPerStep) *msPerStep = best[0] + best[1];
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_set_grids(lenia_batch *b, uint32_t colPerCU, uint32_t rowPerCU) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_set_grids: NULL batch");
+    if (colPerCU > 64 || rowPerCU > 64)
+        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_set_grids: %u / %u workgroups per CU (at most 64)",
+                    colPerCU, rowPerCU);
+    if (!b->isFft && (colPerCU || rowPerCU))
+        return fail(LENIA_ERR_UNSUPPORTED, "lenia_batch_set_grids: tile queues need the FFT form");
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->colGridPerCU = (int)colPerCU;
+    b->rowGridPerCU = (int)rowPerCU;
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);  // captured with the old grids
+    b->graph = nullptr;
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_get_grids(const lenia_batch *b, uint32_t *colPerCU, uint32_t *rowPerCU) {
+    if (!b || !colPerCU || !rowPerCU) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_get_grids: bad argument");
+    *colPerCU = (uint32_t)b->colGridPerCU;
+    *rowPerCU = (uint32_t)b->rowGridPerCU;
     return LENIA_OK;
 }
 

@@ -234,6 +234,17 @@ class Batch:
         check(_abi.lib().lenia_batch_autotune(self._h, n_steps, C.byref(ms)))
         return ms.value
 
+    def set_grids(self, col_per_cu: int, row_per_cu: int):
+        """resident workgroups per CU taking tiles from a queue, per pass
+        (0: one workgroup per tile)"""
+        check(_abi.lib().lenia_batch_set_grids(self._h, col_per_cu, row_per_cu))
+
+    def grids(self) -> tuple[int, int]:
+        """(column, row) workgroups per CU; 0 is one workgroup per tile"""
+        c, r = C.c_uint32(), C.c_uint32()
+        check(_abi.lib().lenia_batch_get_grids(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
+
     def close(self):
         if self._h:
             _abi.lib().lenia_batch_destroy(self._h)

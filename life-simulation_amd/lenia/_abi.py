@@ -35,6 +35,7 @@ EXPORTS = [
     "lenia_world_write_state", "lenia_world_state_device", "lenia_world_potential",
     "lenia_world_get_desc", "lenia_world_pass_times",
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
+    "lenia_batch_set_grids", "lenia_batch_get_grids",
     "lenia_rollout_forward", "lenia_rollout_backward",
     "lenia_world_autotune", "lenia_batch_autotune", "lenia_world_set_organism",
     "lenia_world_save_npy", "lenia_world_load_npy",
@@ -182,6 +183,8 @@ def lib():
         "lenia_rollout_forward": (i32, [vp, u32, u32, vp, vp, i32]),
         "lenia_world_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
+        "lenia_batch_set_grids": (i32, [vp, u32, u32]),
+        "lenia_batch_get_grids": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
         "lenia_world_set_organism": (i32, [vp, fp, u32, u32, u32, u32]),
         "lenia_world_save_npy": (i32, [vp, C.c_char_p]),

@@ -30,7 +30,7 @@ def run_alone(ctx, desc, seed, steps, init=None):
         return w.read()
 
 
-def run_batched(ctx, descs, seeds, steps, graph_steps, inits=None):
+def run_batched(ctx, descs, seeds, steps, graph_steps, inits=None, grids=None):
     import lenia
 
     worlds = [lenia.World(ctx, d) for d in descs]
@@ -41,7 +41,19 @@ def run_batched(ctx, descs, seeds, steps, graph_steps, inits=None):
                 w.write(inits[i])
             w.frame(0, 1)
         with lenia.Batch(ctx, worlds) as b:
-            b.step(steps, graph_steps)
+            if grids is None:
+                b.step(steps, graph_steps)
+            else:
+                # several launch sequences, each with its own grids: the tile
+                # queues must start every launch from zero
+                done = 0
+                for g in grids:
+                    b.set_grids(*g)
+                    assert b.grids() == tuple(g)
+                    n = steps // len(grids)
+                    b.step(n, graph_steps if graph_steps and n % graph_steps == 0 else 0)
+                    done += n
+                b.step(steps - done)
         return [w.read() for w in worlds]
     finally:
         for w in worlds:
@@ -58,6 +70,30 @@ def test_fft_batch_bit_exact_vs_alone(ctx, graph_steps):
         assert got[i].tobytes() == want.tobytes(), (i, d["width"], d["height"])
         print(f"world {i} {d['width']}x{d['height']} K={len(d['kernels'])}: bit-exact, "
               f"mass {float(want.sum()):.3f}")
+
+
+@pytest.mark.parametrize("grids", [[(1, 1)], [(2, 4), (1, 2)], [(4, 1), (0, 0), (1, 0)]])
+def test_fft_batch_tile_queues_bit_exact(ctx, grids):
+    """resident grids taking tiles from the queues (a few workgroups per CU,
+    so each takes many tiles of several worlds), switched between launch
+    sequences and replayed from graphs: the same bytes as each world alone"""
+    descs = sweep_worlds(24, seed=7)
+    seeds = [1000 + i for i in range(len(descs))]
+    got = run_batched(ctx, descs, seeds, 24, 4, grids=grids)
+    for i, d in enumerate(descs):
+        want = run_alone(ctx, d, seeds[i], 24)
+        assert got[i].tobytes() == want.tobytes(), (grids, i, d["width"], d["height"])
+
+
+def test_fft_batch_tile_queues_many_worlds(ctx):
+    """the 256-world sweep with one resident workgroup per CU per pass (more
+    than ten tiles per workgroup), two graph replays"""
+    descs = sweep_worlds(256)
+    seeds = [1000 + i for i in range(256)]
+    got = run_batched(ctx, descs, seeds, 16, 8, grids=[(1, 1)])
+    for i in range(0, 256, 5):
+        want = run_alone(ctx, descs[i], seeds[i], 16)
+        assert got[i].tobytes() == want.tobytes(), (i, descs[i]["width"], descs[i]["height"])
 
 
 def test_direct_batch_bit_exact_vs_alone(ctx):
@@ -126,6 +162,11 @@ def test_batch_errors(ctx):
         with lenia.Batch(ctx, [a]) as bt:
             with pytest.raises(lenia.LeniaError):
                 bt.step(10, 4)  # not a multiple of the graph length
+            with pytest.raises(lenia.LeniaError):
+                bt.set_grids(65, 0)  # more than 64 workgroups per CU
+        with lenia.Batch(ctx, [b]) as bt:
+            with pytest.raises(lenia.LeniaError):
+                bt.set_grids(1, 1)  # tile queues are for the FFT form
     finally:
         a.close()
         b.close()
@@ -175,6 +216,7 @@ def test_sweep_256_as_benchmarked(ctx):
             w.frame(0, 1)
         with lenia.Batch(ctx, worlds) as bt:
             ms = bt.autotune(10)
+            print(f"sweep 256: tuned grids (columns, rows per CU) {bt.grids()}")
             bt.step(50, 50)
             bt.step(200, 50)
             got = [w.read() for w in worlds]
