@@ -65,14 +65,27 @@ def window_nnz(kernel: dict) -> int:
     return int(np.count_nonzero(O.kernel_window(kernel)))
 
 
-def fft_roofline(W: int, H: int, K: int, passes: list) -> dict:
-    """per-pass achieved HBM bandwidth from the mean kernel durations"""
+ENGINE_LDS, ENGINE_WAVE, ENGINE_WAVE_SPLIT = 1, 2, 3
+ENGINE_NAMES = {ENGINE_LDS: "lds", ENGINE_WAVE: "wave", ENGINE_WAVE_SPLIT: "wave-split"}
+
+
+def pass_kernel(name: str, engine: int) -> str:
+    """kernel that runs a step pass on its executor: workgroup-LDS, wave, or
+    kernel-split wave"""
+    if engine == ENGINE_WAVE_SPLIT:
+        return f"k_wave_{name}_split"
+    return ("k_wave_" if engine == ENGINE_WAVE else "k_fft_") + name
+
+
+def fft_roofline(W: int, H: int, K: int, passes: list, engines=(ENGINE_LDS, ENGINE_LDS)) -> dict:
+    """per-pass achieved HBM bandwidth from the mean kernel durations;
+    engines: the (column, row) pass executors"""
     bpc = fft_bytes_per_cell(W, K)
     out = {}
-    for name, ms in zip(("cols", "rows"), passes):
+    for name, ms, eng in zip(("cols", "rows"), passes, engines):
         alg = bpc[name] * W * H
         gbs = alg / (ms * 1e-3) / 1e9
-        out[name] = {"kernel": "k_fft_" + name, "ms": round(ms, 5),
+        out[name] = {"kernel": pass_kernel(name, eng), "ms": round(ms, 5),
                      "algorithmic_bytes_per_launch": int(round(alg)),
                      "algorithmic_bytes_per_cell": round(bpc[name], 3),
                      "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
@@ -199,16 +212,15 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
             ctx.sync()
             batch_wall = time.perf_counter() - t0
             batch_ms = ctx.last_gpu_ms()
-            # the launch form autotune did not pick, timed the same way:
-            # resident grids with tile queues vs one workgroup per tile
-            tuned_grids = b.grids()
-            alt_grids = (0, 0) if tuned_grids != (0, 0) else (1, 1)
-            b.set_grids(*alt_grids)
+            # the same batch with every world on the workgroup-LDS executor,
+            # timed the same way (the executors give the same bits)
+            tuned_waves = b.waves()
+            b.set_waves(0, 0)
             b.step(graph_steps, graph_steps)
             b.step(steps, graph_steps)
             ctx.sync()
-            alt_ms = ctx.last_gpu_ms()
-            b.set_grids(*tuned_grids)
+            lds_ms = ctx.last_gpu_ms()
+            b.set_waves(*tuned_waves)
         batch_rate = cells * steps / (batch_ms * 1e-3) / 1e6
         batch_wall_rate = cells * steps / batch_wall / 1e6
         plain = None
@@ -234,8 +246,8 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
                 "speedup_vs_plain": plain["speedup_batched_vs_plain_wall"] if plain else None,
                 "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                 "bytes_per_step": int(bytes_step), "autotuned_ms_per_step": round(tuned_ms, 5),
-                "grids_per_cu": {"tuned": list(tuned_grids), "alternative": list(alt_grids),
-                                 "alternative_ms_per_step": round(alt_ms / steps, 5)},
+                "executors": {"tuned_waves_rows_cols": list(tuned_waves),
+                              "lds_only_ms_per_step": round(lds_ms / steps, 5)},
                 "radix_plans": plans}
     finally:
         for w in worlds:
@@ -380,7 +392,7 @@ def matrix_bench(ctx, cpu_s: float, tune_plans: bool = True) -> dict:
             ms = ctx.last_gpu_ms()
             passes = w.pass_times(PASS_SAMPLES)
             tiles = w.resolved()
-        roof = fft_roofline(W, H, K, passes)
+        roof = fft_roofline(W, H, K, passes, (tiles["colEngine"], tiles["rowEngine"]))
         dom = max(roof, key=lambda k: roof[k]["ms"])
         step_gbs = fft_bytes_per_cell(W, K)["step"] * W * H / (ms / 500 * 1e-3) / 1e9
         rec = {"workload": f"lenia-fft-{W}x{H}-K3-R24", "steps": 500,
@@ -575,7 +587,7 @@ def main() -> int:
         lifebench = None if args.no_lifebench else lifebench_bench(ctx)
     sec = gpu_ms / 1e3
     value = cells * args.steps / sec / 1e6
-    roof = fft_roofline(W, H, K, passes)
+    roof = fft_roofline(W, H, K, passes, (tiles["colEngine"], tiles["rowEngine"]))
     dom = max(roof, key=lambda k: roof[k]["ms"])
     step_gbs = fft_bytes_per_cell(W, K)["step"] * cells * args.steps / sec / 1e9
     out = {
@@ -597,7 +609,10 @@ def main() -> int:
                    "fftPlanW": tiles["fftPlanW"], "fftPlanH": tiles["fftPlanH"],
                    "rowPairsPerTile": tiles["rowPairsPerTile"],
                    "colsPerTile": tiles["colsPerTile"], "rowThreads": tiles["rowThreads"],
-                   "colThreads": tiles["colThreads"], "autotuned": not args.tiles,
+                   "colThreads": tiles["colThreads"],
+                   "rowEngine": ENGINE_NAMES[tiles["rowEngine"]],
+                   "colEngine": ENGINE_NAMES[tiles["colEngine"]],
+                   "autotuned": not args.tiles,
                    "radix_plans_tuned": not (args.tiles or args.no_plan_tune)},
         "passes": roof,
         "pass_samples": PASS_SAMPLES,

@@ -96,7 +96,22 @@ typedef struct lenia_world_desc {
     uint32_t fftPlanW, fftPlanH;
     uint32_t rowPairsPerTile, colsPerTile;
     uint32_t rowThreads, colThreads;
+    /* Transform executor of each FFT pass (LENIA_ENGINE_*; 0 = library
+     * choice).  Both executors run the same radix plan with the same
+     * butterflies, so the choice never changes a bit.  With the wave
+     * executor the tile fields above are derived: rowThreads / colThreads
+     * (64, 128 or 256) give the waves per workgroup, and a tile holds
+     * 64 / L row pairs or columns per wave (L lanes per transform, set by
+     * the plan); rowPairsPerTile / colsPerTile must then be 0. */
+    uint32_t rowEngine, colEngine;
 } lenia_world_desc;
+
+#define LENIA_ENGINE_AUTO 0 /* no LDS shape given: kernel-split wave columns when the plan has a
+                               wave kernel, LDS rows */
+#define LENIA_ENGINE_LDS 1  /* a workgroup's threads share each stage through LDS barriers */
+#define LENIA_ENGINE_WAVE 2 /* one wave owns whole transforms: registers + wave-private LDS */
+#define LENIA_ENGINE_WAVE_SPLIT 3 /* the wave executor with one wave per kernel transform (4 waves,
+                                     256 threads, per workgroup of 64 / L row pairs or columns) */
 
 typedef struct lenia_ctx lenia_ctx;
 typedef struct lenia_world lenia_world;
@@ -262,15 +277,25 @@ int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
  * pass for all worlds) picked by timing every allowed choice over nSteps steps;
  * states and clocks restored, results unchanged bit for bit */
 int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep);
-/* FFT-form launch grids (no counterpart in the reference, which steps one
- * variant at a time): with perCU > 0 a pass launches min(tiles, perCU x CUs)
- * resident workgroups that take the batch's tiles from a queue, else one
- * workgroup per tile (0, the default).  lenia_batch_autotune times 0, 1, 2
- * and 4 per CU and keeps the fastest.  Queues apply to passes with side-by-
- * side kernel transforms (and, for rows, no noise / jitter / inject terms);
- * other passes stay one workgroup per tile.  Results are identical either way. */
+/* FFT-form launch grids (no counterpart in the reference).  Withdrawn: the
+ * resident tile-queue grids ran 3x slower than one workgroup per tile on the
+ * 256-world sweep.  set_grids accepts 0, 0 (one workgroup per tile, the only
+ * launch form) and refuses anything else with LENIA_ERR_UNSUPPORTED;
+ * get_grids reports 0, 0. */
 int lenia_batch_set_grids(lenia_batch *b, uint32_t colPerCU, uint32_t rowPerCU);
 int lenia_batch_get_grids(const lenia_batch *b, uint32_t *colPerCU, uint32_t *rowPerCU);
+/* FFT-form transform executor of the batch (no counterpart in the reference):
+ * with n > 0 waves per workgroup (1, 2 or 4) the worlds whose radix plan has
+ * a wave kernel run the pass on the wave executor (one launch per values-per-
+ * lane class, the launches of a pass side by side), the others on the
+ * workgroup-LDS one; LENIA_WAVES_PER_KERNEL (3) runs the kernel-split wave
+ * executor (LENIA_ENGINE_WAVE_SPLIT); 0 runs every world on the LDS
+ * executor.  Default 0 rows, 4 columns; lenia_batch_autotune times 0, 1, 2,
+ * 4 and 3.
+ * Every executor computes the same bits. */
+#define LENIA_WAVES_PER_KERNEL 3
+int lenia_batch_set_waves(lenia_batch *b, uint32_t rowWaves, uint32_t colWaves);
+int lenia_batch_get_waves(const lenia_batch *b, uint32_t *rowWaves, uint32_t *colWaves);
 /* nFrames consecutive frames frame0, frame0+1, ... of every world, substeps
  * steps each, with the frame / substep clock of LeniaModule::encode
  * (Lenia.cpp:562-564) -- what one SceneRunner::step per frame does to each

@@ -54,48 +54,6 @@ __device__ __forceinline__ int tile_of(int ti, int nTiles) {
     return gridDim.x == (unsigned)nTiles ? xcd_tile(ti, nTiles) : ti;
 }
 
-// ---------------------------------------------------------------- tile queue
-// Persistent launches (batches): a resident grid takes tiles from the
-// world-ordered tile list instead of one workgroup per tile.  The list is
-// split into 8 contiguous chunks, one per XCD (workgroup ids reach the XCDs
-// round robin), so consecutive tiles of one world -- which share cache lines
-// and the twiddle table -- run on one XCD; a workgroup whose chunk is empty
-// takes from the next ones.  q[x] counts the tiles taken from chunk x and
-// q[8] the workgroups that found every chunk empty; the last of those resets
-// all nine for the next launch.  No workgroup ever waits for another.
-__device__ __forceinline__ int queue_next(int *q, int nTiles) {
-    const int x0 = (int)(blockIdx.x & 7u);
-    const int base = nTiles >> 3, rem = nTiles & 7;
-    for (int k = 0; k < 8; ++k) {
-        const int x = (x0 + k) & 7;
-        const int len = base + (x < rem ? 1 : 0);
-        const int i = atomicAdd(&q[x], 1);
-        if (i < len) return x * base + min(x, rem) + i;
-    }
-    return nTiles;
-}
-
-__device__ __forceinline__ void queue_exit(int *q) {
-    if (atomicAdd(&q[8], 1) == (int)gridDim.x - 1)
-        for (int x = 0; x < 9; ++x) atomicExch(&q[x], 0);
-}
-
-// The queue's next tile for this workgroup (nTiles: none left) and the
-// thread id to use for it.  The tile index is read back through LDS and made
-// wave-uniform, so the world descriptor stays in scalar registers; the thread
-// id passes an empty asm each time, so the values computed from it (the LDS
-// addresses of every FFT stage) are recomputed per tile instead of being
-// hoisted out of the tile loop and held in registers across it.
-__device__ __forceinline__ int queue_tile(int *queue, int nTiles, int *sTile, int &tid) {
-    if (threadIdx.x == 0) *sTile = queue_next(queue, nTiles);
-    __syncthreads();
-    const int ti = __builtin_amdgcn_readfirstlane(*sTile);
-    int t;
-    asm volatile("" : "=v"(t) : "0"((int)threadIdx.x));
-    tid = t;
-    return ti;
-}
-
 // ---------------------------------------------------------------- row pass
 // Twiddle-table copy global -> LDS, split in two so its loads can be in
 // flight together with the other loads of a tile
@@ -400,30 +358,14 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     __syncthreads();
 }
 
-// One tile per workgroup, or (QUEUE, mode 1 only) a resident grid taking
-// tiles from `queue`
-template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T), bool QUEUE = false>
+// One tile per workgroup (grid = tiles)
+template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T)>
 __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles,
                                                 uint32_t stepOff, int kSel,
-                                                float *__restrict__ out, int *queue) {
+                                                float *__restrict__ out) {
     extern __shared__ float2 smem[];
-    if constexpr (QUEUE) {
-        __shared__ int sTile;
-        int prevWorld = -1;
-        for (;;) {
-            int tid;
-            const int ti = queue_tile(queue, nTiles, &sTile, tid);
-            if (ti >= nTiles) break;
-            const int2 tile = tiles[ti];
-            // the twiddle table in LDS still holds this world's when the
-            // previous tile of this workgroup was of the same world
-            rows_tile<MODE, T, EXTRAS, KPAR, V>(worlds, tile, tile.x != prevWorld, stepOff, kSel,
-                                                out, smem, tid);
-            prevWorld = tile.x;
-        }
-        if (threadIdx.x == 0) queue_exit(queue);
-    } else if ((int)blockIdx.x < nTiles) {  // one tile per workgroup (grid = nTiles)
+    if ((int)blockIdx.x < nTiles) {
         const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
         rows_tile<MODE, T, EXTRAS, KPAR, V>(worlds, tile, true, stepOff, kSel, out, smem,
                                             (int)threadIdx.x);
@@ -553,25 +495,12 @@ __device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds
     }
 }
 
-// One tile per workgroup, or (QUEUE) a resident grid taking tiles from `queue`
-template <int T, bool KPAR, int V = kMaxV, bool QUEUE = false>
+// One tile per workgroup (grid = tiles)
+template <int T, bool KPAR, int V = kMaxV>
 __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ worlds,
-                                                const int2 *__restrict__ tiles, int nTiles,
-                                                int *queue) {
+                                                const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
-    if constexpr (QUEUE) {
-        __shared__ int sTile;
-        int prevWorld = -1;
-        for (;;) {
-            int tid;
-            const int ti = queue_tile(queue, nTiles, &sTile, tid);
-            if (ti >= nTiles) break;
-            const int2 tile = tiles[ti];
-            cols_tile<T, KPAR, V>(worlds, tile, tile.x != prevWorld, smem, tid);
-            prevWorld = tile.x;
-        }
-        if (threadIdx.x == 0) queue_exit(queue);
-    } else if ((int)blockIdx.x < nTiles) {  // one tile per workgroup (grid = nTiles)
+    if ((int)blockIdx.x < nTiles) {
         const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
         cols_tile<T, KPAR, V>(worlds, tile, true, smem, (int)threadIdx.x);
     }
@@ -581,74 +510,61 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
 template <int T, bool EX, bool KP, int V = row_values(T)>
 void rows_launch(int mode, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
                         int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                        hipStream_t s, int *queue = nullptr) {
+                        hipStream_t s) {
     if (mode == 0)
         launch_timed(k_fft_rows<0, T, false, false>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, stepOff, kSel, out, queue);
-    else if (mode == 1 && queue) {
-        // tile queues: side-by-side transforms without the extra terms only
-        if constexpr (KP && !EX)
-            launch_timed(k_fft_rows<1, T, false, true, V, true>, dim3(grid), dim3(T), lds, s, worlds,
-                         tiles, nTiles, stepOff, kSel, out, queue);
-    } else if (mode == 1)
+                     nTiles, stepOff, kSel, out);
+    else if (mode == 1)
         launch_timed(k_fft_rows<1, T, EX, KP, V>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles,
-                     stepOff, kSel, out, queue);
+                     stepOff, kSel, out);
     else
         launch_timed(k_fft_rows<2, T, false, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, stepOff, kSel, out, queue);
+                     nTiles, stepOff, kSel, out);
 }
 
 template <int T>
 void rows_dispatch(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
                           const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
-                          int kSel, float *out, hipStream_t s, int *queue) {
+                          int kSel, float *out, hipStream_t s) {
     if (extras) {
-        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
-        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+        if (kpar) rows_launch<T, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, true, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     } else {
-        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
-        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+        if (kpar) rows_launch<T, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<T, false, false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     }
 }
 
 template <int T>
 void cols_dispatch(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,
-                   int nTiles, int grid, size_t lds, hipStream_t s, int *queue) {
-    if (kpar && lean && queue)
-        launch_timed(k_fft_cols<T, true, kLeanCols, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, queue);
-    else if (kpar && queue)
-        launch_timed(k_fft_cols<T, true, kMaxV, true>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, queue);
-    else if (queue)
-        return;  // tile queues: side-by-side transforms only
-    else if (kpar && lean)
+                   int nTiles, int grid, size_t lds, hipStream_t s) {
+    if (kpar && lean)
         launch_timed(k_fft_cols<T, true, kLeanCols>, dim3(grid), dim3(T), lds, s, worlds, tiles,
-                     nTiles, queue);
+                     nTiles);
     else if (kpar)
-        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, queue);
+        launch_timed(k_fft_cols<T, true>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
     else
-        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, queue);
+        launch_timed(k_fft_cols<T, false>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles);
 }
 
 // per-workgroup-size entry points (k_fft_pass<T>.hip)
 #define LENIA_FFT_PASS_DECL(T)                                                                  \
     void fft_rows_pass_##T(int mode, bool extras, bool kpar, const FftWorldDev *worlds,         \
                            const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff, \
-                           int kSel, float *out, hipStream_t s, int *queue);                    \
+                           int kSel, float *out, hipStream_t s);                    \
     void fft_cols_pass_##T(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,   \
-                           int nTiles, int grid, size_t lds, hipStream_t s, int *queue);
+                           int nTiles, int grid, size_t lds, hipStream_t s);
 LENIA_FFT_PASS_DECL(64)
 LENIA_FFT_PASS_DECL(128)
 LENIA_FFT_PASS_DECL(256)
 LENIA_FFT_PASS_DECL(512)
 // 256- / 512-thread step passes holding kMidValues cells per thread (side-by-side transforms)
 void fft_rows_step_256_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t stepOff, hipStream_t s, int *queue);
+                           int grid, size_t lds, uint32_t stepOff, hipStream_t s);
 void fft_rows_step_512_mid(bool extras, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
-                           int grid, size_t lds, uint32_t stepOff, hipStream_t s, int *queue);
+                           int grid, size_t lds, uint32_t stepOff, hipStream_t s);
 void fft_rows_pass_1024(int mode, bool extras, bool lean, const FftWorldDev *worlds, const int2 *tiles,
                         int nTiles, int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                        hipStream_t s, int *queue);
+                        hipStream_t s);
 
 }  // namespace lenia

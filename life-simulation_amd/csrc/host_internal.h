@@ -38,6 +38,13 @@ struct lenia_world {
     int P = 1, C = 1;       // row pairs / columns per tile
     int rowT = 256, colT = 256;  // workgroup sizes
     bool rowKpar = false, colKpar = false;  // K inverse transforms side by side
+    // wave executor per pass (else the workgroup-LDS one) and its waves per workgroup
+    bool rowWave = false, colWave = false;
+    int rowNW = 1, colNW = 1;
+    // kernel-split wave passes (one wave per kernel transform, kMaxKernels
+    // waves per workgroup); the row pass's forward-only and potential passes
+    // then run the one-wave kernels on the same tiles
+    bool rowSplit = false, colSplit = false;
     hipGraphExec_t graphExec = nullptr;     // captured step sequence
     bool graphExtras = false;
     uint32_t version = 0;  // bumped on every descriptor upload (batches re-read)
@@ -107,6 +114,15 @@ bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
 bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
                 int &ntOut, int &TOut, bool &kparOut);
 hipError_t enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t off);
+// a world's row pass (mode 0 forward, 1 step, 2 potential of kernel kSel into
+// out) and column pass on its chosen executor; `rec` is the device record to
+// run (the world's own, or a copy with the reverse pass's save pointers)
+hipError_t world_launch_rows(lenia_world *w, int mode, const FftWorldDev *rec, uint32_t off,
+                             int kSel, float *out, bool extras, hipStream_t s);
+hipError_t world_launch_cols(lenia_world *w, const FftWorldDev *rec, hipStream_t s);
+// default waves per workgroup of the wave executor for `count` transforms of
+// L lanes: the largest of 4, 2, 1 that still gives every CU a workgroup
+int wave_default_nw(int count, int L, int numCUs);
 hipError_t world_advance_clock(lenia_world *w, uint32_t by);
 hipError_t enqueue_steps(lenia_world *w, uint32_t n);
 int world_ensure_spectrum(lenia_world *w);

@@ -72,30 +72,28 @@ __global__ void k_spec_cols(const double *__restrict__ Cw, int h, int H, int Wh,
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                            size_t lds, uint32_t stepOff, int kSel, float *out,
-                           hipStream_t s, int tileCells, int *queue) {
-    // one tile per workgroup, or a resident grid taking tiles from a queue
-    // (step passes with side-by-side transforms and no extra terms)
-    if (grid < 1 || (grid < nTiles && !queue)) return hipErrorInvalidValue;
-    if (queue && (mode != 1 || extras || !kpar)) return hipErrorInvalidValue;
+                           hipStream_t s, int tileCells) {
+    // one tile per workgroup
+    if (grid < 1 || grid < nTiles) return hipErrorInvalidValue;
     switch (threads) {
-        case 64: fft_rows_pass_64(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue); break;
-        case 128: fft_rows_pass_128(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue); break;
+        case 64: fft_rows_pass_64(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
+        case 128: fft_rows_pass_128(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s); break;
         case 256:
             if (mode == 1 && kpar && tileCells > 0 && tileCells <= kMidValues * 256)
-                fft_rows_step_256_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s, queue);
+                fft_rows_step_256_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s);
             else
-                fft_rows_pass_256(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+                fft_rows_pass_256(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
             break;
         case 512:
             if (mode == 1 && kpar && tileCells > 0 && tileCells <= kMidValues * 512)
-                fft_rows_step_512_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s, queue);
+                fft_rows_step_512_mid(extras, worlds, tiles, nTiles, grid, lds, stepOff, s);
             else
-                fft_rows_pass_512(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+                fft_rows_pass_512(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
             break;
         case 1024:  // steps with side-by-side transforms only (the serial form needs more registers)
             if (mode == 1 && !kpar) return hipErrorInvalidValue;
             fft_rows_pass_1024(mode, extras, tileCells > 0 && tileCells <= kLeanValues * 1024, worlds,
-                               tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+                               tiles, nTiles, grid, lds, stepOff, kSel, out, s);
             break;
         default: return hipErrorInvalidValue;
     }
@@ -103,14 +101,14 @@ hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
 }
 
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
-                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells, int *queue) {
-    if (grid < 1 || (grid < nTiles && !queue) || (queue && !kpar)) return hipErrorInvalidValue;
+                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells) {
+    if (grid < 1 || grid < nTiles) return hipErrorInvalidValue;
     const bool lean = tileCells > 0 && tileCells <= kLeanCols * threads;
     switch (threads) {
-        case 64: fft_cols_pass_64(kpar, lean, worlds, tiles, nTiles, grid, lds, s, queue); break;
-        case 128: fft_cols_pass_128(kpar, lean, worlds, tiles, nTiles, grid, lds, s, queue); break;
-        case 256: fft_cols_pass_256(kpar, lean, worlds, tiles, nTiles, grid, lds, s, queue); break;
-        case 512: fft_cols_pass_512(kpar, lean, worlds, tiles, nTiles, grid, lds, s, queue); break;
+        case 64: fft_cols_pass_64(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 128: fft_cols_pass_128(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 256: fft_cols_pass_256(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
+        case 512: fft_cols_pass_512(kpar, lean, worlds, tiles, nTiles, grid, lds, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -8,14 +8,14 @@ namespace lenia {
 // lean: every tile holds at most kLeanValues cells per thread
 void fft_rows_pass_1024(int mode, bool extras, bool lean, const FftWorldDev *worlds,
                         const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
-                        int kSel, float *out, hipStream_t s, int *queue) {
+                        int kSel, float *out, hipStream_t s) {
     if (lean && mode == 1) {
-        if (extras) rows_launch<1024, true, true, kLeanValues>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
-        else rows_launch<1024, false, true, kLeanValues>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+        if (extras) rows_launch<1024, true, true, kLeanValues>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else rows_launch<1024, false, true, kLeanValues>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
         return;
     }
-    if (extras) rows_launch<1024, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
-    else rows_launch<1024, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+    if (extras) rows_launch<1024, true, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+    else rows_launch<1024, false, true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
 }
 
 }  // namespace lenia

@@ -5,13 +5,13 @@ namespace lenia {
 
 void fft_rows_pass_128(int mode, bool extras, bool kpar, const FftWorldDev *worlds,
                        const int2 *tiles, int nTiles, int grid, size_t lds, uint32_t stepOff,
-                       int kSel, float *out, hipStream_t s, int *queue) {
-    rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s, queue);
+                       int kSel, float *out, hipStream_t s) {
+    rows_dispatch<128>(mode, extras, kpar, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
 }
 
 void fft_cols_pass_128(bool kpar, bool lean, const FftWorldDev *worlds, const int2 *tiles,
-                       int nTiles, int grid, size_t lds, hipStream_t s, int *queue) {
-    cols_dispatch<128>(kpar, lean, worlds, tiles, nTiles, grid, lds, s, queue);
+                       int nTiles, int grid, size_t lds, hipStream_t s) {
+    cols_dispatch<128>(kpar, lean, worlds, tiles, nTiles, grid, lds, s);
 }
 
 }  // namespace lenia

@@ -37,11 +37,30 @@ inline void launch_timed(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_
 hipError_t launch_fft_rows(int mode, int threads, bool extras, bool kpar,
                            const FftWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                            size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s,
-                           int tileCells = 0, int *queue = nullptr);
+                           int tileCells = 0);
 // tileCells: the largest (K) x nt x H over the launch's tiles, or 0
 hipError_t launch_fft_cols(int threads, bool kpar, const FftWorldDev *worlds, const int2 *tiles,
-                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells = 0,
-                           int *queue = nullptr);
+                           int nTiles, int grid, size_t lds, hipStream_t s, int tileCells = 0);
+// Wave-resident executor of the same passes (k_fft_wave.hip): nw waves per
+// workgroup, each running 64 / L transforms of L lanes; a tile is nw (64 / L)
+// row pairs or columns.  Only for plans with wave_lanes(code) > 0; a launch
+// covers the plans of one values-per-lane class vc = N / L (wave_class).
+int wave_lanes(uint32_t planCode);
+int wave_class(uint32_t planCode, int n);
+size_t wave_lds_bytes(int N, int L, int nw);
+hipError_t launch_wave_rows(int mode, int nw, int vc, bool extras, const FftWorldDev *worlds,
+                            const int2 *tiles, int nTiles, size_t lds, uint32_t stepOff, int kSel,
+                            float *out, hipStream_t s);
+hipError_t launch_wave_cols(int nw, int vc, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                            size_t lds, hipStream_t s);
+// kernel-split step passes: kMaxKernels waves per workgroup, one per kernel
+// transform (the row pass's forward transform and the column pass's forward
+// transform on wave 0); a tile is 64 / L row pairs or columns
+size_t wave_split_lds_bytes(int N, int L, bool rows);
+hipError_t launch_wave_rows_split(bool extras, int vc, const FftWorldDev *worlds, const int2 *tiles,
+                                  int nTiles, size_t lds, uint32_t stepOff, hipStream_t s);
+hipError_t launch_wave_cols_split(int vc, const FftWorldDev *worlds, const int2 *tiles, int nTiles,
+                                  size_t lds, hipStream_t s);
 hipError_t launch_twiddles(float *tw, int N, hipStream_t s);
 hipError_t launch_stage_twiddles(const FftPlan &plan, const float *raw, float *out, hipStream_t s);
 hipError_t launch_kernel_spectrum(const double *win, int h, int W, int H, double *Cw, float *S,

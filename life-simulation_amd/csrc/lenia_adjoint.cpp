@@ -264,9 +264,7 @@ static uint32_t ckpt_spacing(uint32_t T, uint32_t ckptEvery) {
 static hipError_t load_state(lenia_world *w, const float *src, hipStream_t s) {
     hipError_t e = hipMemcpyAsync(w->current_state(), src, sizeof(float) * w->adj->n,
                                   hipMemcpyDeviceToDevice, s);
-    if (e == hipSuccess)
-        e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles,
-                            w->nRowTiles, w->nRowTiles, w->ldsRows, 0, 0, nullptr, s);
+    if (e == hipSuccess) e = world_launch_rows(w, 0, (const FftWorldDev *)w->devDesc, 0, 0, nullptr, false, s);
     return e;
 }
 
@@ -298,11 +296,8 @@ static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
     const FftWorldDev *recs = ws->recs + (size_t)(sg % 2) * c;
     hipError_t e = load_state(w, ws->ckpt + ws->n * sg, s);
     for (uint32_t j = 0; j < m && e == hipSuccess; ++j) {
-        e = launch_fft_cols(w->colT, w->colKpar, recs + j, w->colTiles, w->nColTiles, w->nColTiles,
-                            w->ldsCols, s, w->col_tile_cells());
-        if (e == hipSuccess)
-            e = launch_fft_rows(1, w->rowT, false, w->rowKpar, recs + j, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, 0, 0, nullptr, s, w->row_tile_cells());
+        e = world_launch_cols(w, recs + j, s);
+        if (e == hipSuccess) e = world_launch_rows(w, 1, recs + j, 0, 0, nullptr, false, s);
     }
     return e;
 }

@@ -39,13 +39,22 @@ struct lenia_batch {
     // launch shapes fixed by lenia_batch_autotune (0: the default choice)
     int tuneRowT = 0, tuneColT = 0;
     bool tuneRowKpar = true, tuneColKpar = true;
-    // resident grids of workgroups per CU for the column / row pass, which
-    // then take tiles from a queue (0: one workgroup per tile)
-    int colGridPerCU = 0, rowGridPerCU = 0;
-    // tile-queue counters: 9 for the column pass at 0, 9 for the row pass at
-    // kRowQueue; each launch leaves its nine at zero
-    int *queue = nullptr;
-    static constexpr int kRowQueue = 16;
+    // wave executor: waves per workgroup of each pass (0: every world runs
+    // the workgroup-LDS executor).  Worlds whose plan has a wave kernel run
+    // it, one launch per values-per-lane class; the others form the LDS
+    // group, whose tiles come first in the tile list (nRowTiles / nColTiles
+    // of them).
+    int rowWaveNW = 0, colWaveNW = 4;
+    // the launches of one pass (LDS group, wave classes) run side by side:
+    // forked from the context stream onto these streams and joined back
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t evFork = nullptr, evJoin[kSide] = {};
+    struct WaveGroup {
+        int vc = 0, first = 0, count = 0;
+        size_t lds = 0;
+    };
+    std::vector<WaveGroup> rowWaves, colWaves;
 };
 
 static void batch_release_tables(lenia_batch *b) {
@@ -54,14 +63,40 @@ static void batch_release_tables(lenia_batch *b) {
     dev_free(b->ctx, b->devWorlds);
     dev_free(b->ctx, b->rowTiles);
     dev_free(b->ctx, b->colTiles);
-    dev_free(b->ctx, b->queue);
     b->devWorlds = nullptr;
     b->rowTiles = b->colTiles = nullptr;
-    b->queue = nullptr;
     b->ldsRows = b->ldsCols = 0;
 }
 
-static void batch_release(lenia_batch *b) { batch_release_tables(b); }
+static void batch_release(lenia_batch *b) {
+    batch_release_tables(b);
+    for (int i = 0; i < lenia_batch::kSide; ++i) {
+        if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+        if (b->evJoin[i]) (void)hipEventDestroy(b->evJoin[i]);
+        b->side[i] = nullptr;
+        b->evJoin[i] = nullptr;
+    }
+    if (b->evFork) (void)hipEventDestroy(b->evFork);
+    b->evFork = nullptr;
+}
+
+// launch(i, stream) for i < n: launch 0 on the context stream, the others on
+// side streams that wait for everything before them and that the context
+// stream waits for afterwards (under graph capture: parallel branches)
+template <class F>
+static hipError_t batch_fanout(lenia_batch *b, int n, F launch) {
+    hipStream_t s = b->ctx->stream;
+    if (n <= 1) return n == 1 ? launch(0, s) : hipSuccess;
+    if (n > lenia_batch::kSide + 1) return hipErrorInvalidValue;
+    hipError_t e = hipEventRecord(b->evFork, s);
+    for (int i = 1; i < n && e == hipSuccess; ++i) e = hipStreamWaitEvent(b->side[i - 1], b->evFork, 0);
+    for (int i = 0; i < n && e == hipSuccess; ++i) e = launch(i, i ? b->side[i - 1] : s);
+    for (int i = 1; i < n && e == hipSuccess; ++i) {
+        e = hipEventRecord(b->evJoin[i - 1], b->side[i - 1]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, b->evJoin[i - 1], 0);
+    }
+    return e;
+}
 
 static int batch_setup_fft(lenia_batch *b);
 static int batch_setup_direct(lenia_batch *b);
@@ -84,11 +119,17 @@ static int batch_nt(const FftPlan &p, bool rows, int count, int K, int T, bool k
     return shape_ok(p, rows, nt, K, T, kpar) ? nt : 0;
 }
 
+// lanes per transform of a world's pass on the wave executor, 0: LDS executor
+static int batch_wave_lanes(const lenia_batch *b, const lenia_world *w, bool rows) {
+    const int nw = rows ? b->rowWaveNW : b->colWaveNW;
+    return nw ? wave_lanes(rows ? w->planW.code : w->planH.code) : 0;
+}
+
 static int batch_setup_fft(lenia_batch *b) {
     lenia_ctx *ctx = b->ctx;
     const int n = (int)b->worlds.size();
-    // one workgroup size per pass for the whole batch; side-by-side kernels
-    // when every world allows it (or the tuned shapes)
+    // one workgroup size per pass for the LDS group; side-by-side kernels
+    // when every world of it allows it (or the tuned shapes)
     const int Ts[4] = {256, 512, 128, 64};
     bool found = false;
     for (int kp = 1; kp >= 0 && !found; --kp)
@@ -97,7 +138,8 @@ static int batch_setup_fft(lenia_batch *b) {
             bool ok = true;
             for (lenia_world *w : b->worlds) {
                 const int K = (int)w->desc.nKernels;
-                if (!batch_nt(w->planW, true, w->desc.height / 2, K, T, kp != 0)) ok = false;
+                if (!batch_wave_lanes(b, w, true) && !batch_nt(w->planW, true, w->desc.height / 2, K, T, kp != 0))
+                    ok = false;
             }
             if (ok) {
                 b->rowT = T;
@@ -114,7 +156,8 @@ static int batch_setup_fft(lenia_batch *b) {
             bool ok = true;
             for (lenia_world *w : b->worlds) {
                 const int K = (int)w->desc.nKernels;
-                if (!batch_nt(w->planH, false, w->desc.width / 2 + 1, K, T, kp != 0)) ok = false;
+                if (!batch_wave_lanes(b, w, false) && !batch_nt(w->planH, false, w->desc.width / 2 + 1, K, T, kp != 0))
+                    ok = false;
             }
             if (ok) {
                 b->colT = T;
@@ -126,38 +169,74 @@ static int batch_setup_fft(lenia_batch *b) {
     if (!found) return fail(LENIA_ERR_UNSUPPORTED, "batch: no common column-pass shape");
 
     std::vector<FftWorldDev> table(n);
-    std::vector<int2> rt, ct;
+    // tiles: the LDS group first, then one run per wave class
+    const int kClasses[3] = {8, 16, 24};
+    std::vector<int2> rt, ct, rtw[3], ctw[3];
+    size_t rlds[3] = {0, 0, 0}, clds[3] = {0, 0, 0};
     b->rowCells = b->colCells = 0;
     for (int i = 0; i < n; ++i) {
         lenia_world *w = b->worlds[i];
         FftWorldDev f = w->fdev;  // same buffers, plans, clock; batch tile shape
         const int K = (int)w->desc.nKernels;
-        f.P = batch_nt(w->planW, true, f.H / 2, K, b->rowT, b->rowKpar);
-        f.C = batch_nt(w->planH, false, f.Wh, K, b->colT, b->colKpar);
+        const int Lr = batch_wave_lanes(b, w, true), Lc = batch_wave_lanes(b, w, false);
+        // kernel-split passes (LENIA_WAVES_PER_KERNEL): one group set per workgroup
+        const bool rs = b->rowWaveNW == LENIA_WAVES_PER_KERNEL, cs = b->colWaveNW == LENIA_WAVES_PER_KERNEL;
+        f.P = Lr ? (rs ? 1 : b->rowWaveNW) * (64 / Lr) : batch_nt(w->planW, true, f.H / 2, K, b->rowT, b->rowKpar);
+        f.C = Lc ? (cs ? 1 : b->colWaveNW) * (64 / Lc) : batch_nt(w->planH, false, f.Wh, K, b->colT, b->colKpar);
         f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
         f.colTiles = (f.Wh + f.C - 1) / f.C;
         f.mP = f.P <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + f.P - 1) / f.P);
-        f.rowThreads = b->rowT;
-        f.colThreads = b->colT;
+        f.rowThreads = Lr ? 64 * (rs ? kMaxKernels : b->rowWaveNW) : b->rowT;
+        f.colThreads = Lc ? 64 * (cs ? kMaxKernels : b->colWaveNW) : b->colT;
         table[i] = f;
-        for (int t = 0; t < f.rowTiles; ++t) rt.push_back(make_int2(i, t));
-        for (int t = 0; t < f.colTiles; ++t) ct.push_back(make_int2(i, t));
-        b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar), lds_rows(f.W, f.P, 1, true)});
-        b->ldsCols = std::max(b->ldsCols, lds_cols(f.H, f.C, K, b->colKpar));
-        b->rowCells = std::max(b->rowCells, (b->rowKpar ? K : 1) * f.P * f.W);
-        b->colCells = std::max(b->colCells, (b->colKpar ? K : 1) * f.C * f.H);
+        if (Lr) {
+            const int c = (f.W / Lr) / 8 - 1;  // class 8, 16, 24 -> 0, 1, 2
+            for (int t = 0; t < f.rowTiles; ++t) rtw[c].push_back(make_int2(i, t));
+            rlds[c] = std::max(rlds[c], rs ? wave_split_lds_bytes(f.W, Lr, true) : wave_lds_bytes(f.W, Lr, b->rowWaveNW));
+        } else {
+            for (int t = 0; t < f.rowTiles; ++t) rt.push_back(make_int2(i, t));
+            b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar), lds_rows(f.W, f.P, 1, true)});
+            b->rowCells = std::max(b->rowCells, (b->rowKpar ? K : 1) * f.P * f.W);
+        }
+        if (Lc) {
+            const int c = (f.H / Lc) / 8 - 1;
+            for (int t = 0; t < f.colTiles; ++t) ctw[c].push_back(make_int2(i, t));
+            clds[c] = std::max(clds[c], cs ? wave_split_lds_bytes(f.H, Lc, false) : wave_lds_bytes(f.H, Lc, b->colWaveNW));
+        } else {
+            for (int t = 0; t < f.colTiles; ++t) ct.push_back(make_int2(i, t));
+            b->ldsCols = std::max(b->ldsCols, lds_cols(f.H, f.C, K, b->colKpar));
+            b->colCells = std::max(b->colCells, (b->colKpar ? K : 1) * f.C * f.H);
+        }
     }
     b->nRowTiles = (int)rt.size();
     b->nColTiles = (int)ct.size();
+    b->rowWaves.clear();
+    b->colWaves.clear();
+    for (int c = 0; c < 3; ++c) {
+        if (!rtw[c].empty()) {
+            lenia_batch::WaveGroup gr;
+            gr.vc = kClasses[c];
+            gr.first = (int)rt.size();
+            gr.count = (int)rtw[c].size();
+            gr.lds = rlds[c];
+            b->rowWaves.push_back(gr);
+            rt.insert(rt.end(), rtw[c].begin(), rtw[c].end());
+        }
+        if (!ctw[c].empty()) {
+            lenia_batch::WaveGroup gr;
+            gr.vc = kClasses[c];
+            gr.first = (int)ct.size();
+            gr.count = (int)ctw[c].size();
+            gr.lds = clds[c];
+            b->colWaves.push_back(gr);
+            ct.insert(ct.end(), ctw[c].begin(), ctw[c].end());
+        }
+    }
     int rc = dev_alloc(ctx, &b->devWorlds, sizeof(FftWorldDev) * n);
     if (!rc) rc = dev_alloc(ctx, (void **)&b->rowTiles, sizeof(int2) * rt.size());
     if (!rc) rc = dev_alloc(ctx, (void **)&b->colTiles, sizeof(int2) * ct.size());
-    if (!rc) rc = dev_alloc(ctx, (void **)&b->queue, sizeof(int) * 2 * lenia_batch::kRowQueue);
     if (rc) return rc;
-    hipError_t e = launch_fill(reinterpret_cast<float *>(b->queue), 2 * lenia_batch::kRowQueue, 0.0f,
-                               ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(FftWorldDev) * n,
+    hipError_t e = hipMemcpyAsync(b->devWorlds, table.data(), sizeof(FftWorldDev) * n,
                            hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->rowTiles, rt.data(), sizeof(int2) * rt.size(), hipMemcpyHostToDevice,
@@ -213,6 +292,16 @@ extern "C" int lenia_batch_create(lenia_ctx *ctx, lenia_world *const *worlds, ui
         return fail(LENIA_ERR_UNSUPPORTED, "lenia_batch_create: MFMA forms are stepped alone");
     lenia_batch *b = new lenia_batch();
     b->ctx = ctx;
+    hipError_t es = hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming);
+    for (int i = 0; i < lenia_batch::kSide && es == hipSuccess; ++i) {
+        es = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+        if (es == hipSuccess) es = hipEventCreateWithFlags(&b->evJoin[i], hipEventDisableTiming);
+    }
+    if (es != hipSuccess) {
+        batch_release(b);
+        delete b;
+        return hip_fail(es, "batch streams");
+    }
     b->worlds.assign(worlds, worlds + n);
     b->isFft = worlds[0]->isFft;
     for (lenia_world *w : b->worlds) b->extras = b->extras || w->extras();
@@ -236,32 +325,49 @@ extern "C" int lenia_batch_destroy(lenia_batch *b) {
 }
 
 // one step of every world, `off` steps after its stored clock; with `ev`,
-// kernel p records ev[2p] and ev[2p+1]
+// pass p (columns, rows) records ev[2p] and ev[2p+1] around its launches
 static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint32_t off) {
     lenia_ctx *ctx = b->ctx;
-    const int n = (int)b->worlds.size();
-    hipError_t e;
+    hipStream_t s = ctx->stream;
+    const FftWorldDev *table = (const FftWorldDev *)b->devWorlds;
+    hipError_t e = hipSuccess;
     if (b->isFft) {
-        // a resident grid smaller than the tile list takes tiles from a queue
-        const int cg = b->colKpar ? std::min(b->nColTiles, b->colGridPerCU * ctx->numCUs) : 0;
-        const int rg = b->rowKpar && !b->extras ? std::min(b->nRowTiles, b->rowGridPerCU * ctx->numCUs) : 0;
-        const bool cq = cg > 0 && cg < b->nColTiles, rq = rg > 0 && rg < b->nRowTiles;
-        if (ev) launch_events() = {ev[0], ev[1]};
-        e = launch_fft_cols(b->colT, b->colKpar, (const FftWorldDev *)b->devWorlds, b->colTiles,
-                            b->nColTiles, cq ? cg : b->nColTiles, b->ldsCols, ctx->stream,
-                            b->colCells, cq ? b->queue : nullptr);
-        if (ev) launch_events() = {ev[2], ev[3]};
+        // launch 0: the LDS group when it has tiles; then the wave classes
+        const int cl = b->nColTiles ? 1 : 0, rl = b->nRowTiles ? 1 : 0;
+        if (ev) e = hipEventRecord(ev[0], s);
         if (e == hipSuccess)
-            e = launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, (const FftWorldDev *)b->devWorlds,
-                                b->rowTiles, b->nRowTiles, rq ? rg : b->nRowTiles, b->ldsRows, off, 0,
-                                nullptr, ctx->stream, b->rowCells,
-                                rq ? b->queue + lenia_batch::kRowQueue : nullptr);
+            e = batch_fanout(b, cl + (int)b->colWaves.size(), [&](int i, hipStream_t st) {
+                if (i < cl)
+                    return launch_fft_cols(b->colT, b->colKpar, table, b->colTiles, b->nColTiles,
+                                           b->nColTiles, b->ldsCols, st, b->colCells);
+                const auto &gr = b->colWaves[i - cl];
+                if (b->colWaveNW == LENIA_WAVES_PER_KERNEL)
+                    return launch_wave_cols_split(gr.vc, table, b->colTiles + gr.first, gr.count, gr.lds, st);
+                return launch_wave_cols(b->colWaveNW, gr.vc, table, b->colTiles + gr.first, gr.count,
+                                        gr.lds, st);
+            });
+        if (ev && e == hipSuccess) e = hipEventRecord(ev[1], s);
+        if (ev && e == hipSuccess) e = hipEventRecord(ev[2], s);
+        if (e == hipSuccess)
+            e = batch_fanout(b, rl + (int)b->rowWaves.size(), [&](int i, hipStream_t st) {
+                if (i < rl)
+                    return launch_fft_rows(1, b->rowT, b->extras, b->rowKpar, table, b->rowTiles,
+                                           b->nRowTiles, b->nRowTiles, b->ldsRows, off, 0, nullptr, st,
+                                           b->rowCells);
+                const auto &gr = b->rowWaves[i - rl];
+                if (b->rowWaveNW == LENIA_WAVES_PER_KERNEL)
+                    return launch_wave_rows_split(b->extras, gr.vc, table, b->rowTiles + gr.first, gr.count,
+                                                  gr.lds, off, st);
+                return launch_wave_rows(1, b->rowWaveNW, gr.vc, b->extras, table, b->rowTiles + gr.first,
+                                        gr.count, gr.lds, off, 0, nullptr, st);
+            });
+        if (ev && e == hipSuccess) e = hipEventRecord(ev[3], s);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                          b->nRowTiles, b->ldsRows, off, 0, nullptr, ctx->stream);
+                          b->nRowTiles, b->ldsRows, off, 0, nullptr, s);
+        launch_events() = LaunchEvents();
     }
-    launch_events() = LaunchEvents();
     return e;
 }
 
@@ -357,10 +463,10 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     double best[2] = {1e30, 1e30};
     int bestRowT = 0, bestColT = 0;
     bool bestRowKp = true, bestColKp = true;
-    // columns keep their default while the row shapes are timed, and back;
-    // both passes one workgroup per tile until the grids are timed
+    // columns keep their default while the row shapes are timed, and back
     b->tuneColT = 0;
-    b->colGridPerCU = b->rowGridPerCU = 0;
+    // the workgroup-LDS executor for every world first, the wave executor last
+    b->rowWaveNW = b->colWaveNW = 0;
     for (int kp = 1; kp >= 0 && !rc; --kp)
         for (int T = 64; T <= 512 && !rc; T *= 2) {
             b->tuneRowT = T;
@@ -389,25 +495,26 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
         }
     b->tuneColT = bestColT;
     b->tuneColKpar = bestColKp;
-    // then resident grids with tile queues against one workgroup per tile
-    // (a row: its best grid so far, ms[1]; columns: ms[0])
-    b->colGridPerCU = b->rowGridPerCU = 0;
-    int bestColGrid = 0, bestRowGrid = 0;
-    for (int gpc = 1; gpc <= 4 && !rc && bestRowT && bestColT; gpc *= 2) {
-        b->colGridPerCU = b->rowGridPerCU = gpc;
+    // then the wave executor (worlds whose plan has a wave kernel) at 1, 2
+    // and 4 waves per workgroup and kernel-split, against the best so far
+    int bestRowNW = 0, bestColNW = 0;
+    const int kNw[4] = {1, 2, 4, LENIA_WAVES_PER_KERNEL};
+    for (int ni = 0; ni < 4 && !rc && bestRowT && bestColT; ++ni) {
+        const int nw = kNw[ni];
+        b->rowWaveNW = b->colWaveNW = nw;
         double ms[2];
         if (time_now(ms) != LENIA_OK) continue;
         if (ms[0] < best[0]) {
             best[0] = ms[0];
-            bestColGrid = gpc;
+            bestColNW = nw;
         }
         if (ms[1] < best[1]) {
             best[1] = ms[1];
-            bestRowGrid = gpc;
+            bestRowNW = nw;
         }
     }
-    b->colGridPerCU = bestColGrid;
-    b->rowGridPerCU = bestRowGrid;
+    b->rowWaveNW = bestRowNW;
+    b->colWaveNW = bestColNW;
     const int rc2 = batch_setup(b);
     // restore
     off = 0;
@@ -433,26 +540,49 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     return LENIA_OK;
 }
 
+// The tile-queue grids (a resident grid taking tiles from per-XCD queues) are
+// withdrawn: they ran 3x slower than one workgroup per tile on the 256-world
+// sweep (0.57 against 0.19 ms per step) and the autotuner never chose them.
+// The calls remain for callers built against them: 0, 0 is accepted (one
+// workgroup per tile, the only launch form), anything else is refused.
 extern "C" int lenia_batch_set_grids(lenia_batch *b, uint32_t colPerCU, uint32_t rowPerCU) {
     if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_set_grids: NULL batch");
-    if (colPerCU > 64 || rowPerCU > 64)
-        return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_set_grids: %u / %u workgroups per CU (at most 64)",
-                    colPerCU, rowPerCU);
-    if (!b->isFft && (colPerCU || rowPerCU))
-        return fail(LENIA_ERR_UNSUPPORTED, "lenia_batch_set_grids: tile queues need the FFT form");
+    if (colPerCU || rowPerCU)
+        return fail(LENIA_ERR_UNSUPPORTED,
+                    "lenia_batch_set_grids: tile-queue grids are withdrawn (one workgroup per tile only)");
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_set_waves(lenia_batch *b, uint32_t rowWaves, uint32_t colWaves) {
+    if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_set_waves: NULL batch");
+    auto ok = [](uint32_t v) { return v <= 4; };
+    if (!ok(rowWaves) || !ok(colWaves))
+        return fail(LENIA_ERR_INVALID_ARG,
+                    "lenia_batch_set_waves: %u / %u (0, 1, 2, 4 waves per workgroup or 3, kernel-split)",
+                    rowWaves, colWaves);
+    if (!b->isFft && (rowWaves || colWaves))
+        return fail(LENIA_ERR_UNSUPPORTED, "lenia_batch_set_waves: the wave executor needs the FFT form");
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    b->colGridPerCU = (int)colPerCU;
-    b->rowGridPerCU = (int)rowPerCU;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);  // captured with the old grids
-    b->graph = nullptr;
+    b->rowWaveNW = (int)rowWaves;
+    b->colWaveNW = (int)colWaves;
+    if (b->isFft) {
+        const int rc = batch_setup(b);
+        if (rc) return rc;
+    }
+    return LENIA_OK;
+}
+
+extern "C" int lenia_batch_get_waves(const lenia_batch *b, uint32_t *rowWaves, uint32_t *colWaves) {
+    if (!b || !rowWaves || !colWaves) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_get_waves: bad argument");
+    *rowWaves = (uint32_t)b->rowWaveNW;
+    *colWaves = (uint32_t)b->colWaveNW;
     return LENIA_OK;
 }
 
 extern "C" int lenia_batch_get_grids(const lenia_batch *b, uint32_t *colPerCU, uint32_t *rowPerCU) {
     if (!b || !colPerCU || !rowPerCU) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_get_grids: bad argument");
-    *colPerCU = (uint32_t)b->colGridPerCU;
-    *rowPerCU = (uint32_t)b->rowGridPerCU;
+    *colPerCU = *rowPerCU = 0;
     return LENIA_OK;
 }
 

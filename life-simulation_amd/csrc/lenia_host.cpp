@@ -630,19 +630,71 @@ bool lenia::pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t w
     return false;
 }
 
+int lenia::wave_default_nw(int count, int L, int numCUs) {
+    const int waves = (count + 64 / L - 1) / (64 / L);
+    for (int nw = 4; nw > 1; nw /= 2)
+        if (waves >= nw * numCUs) return nw;
+    return 1;
+}
+
+// Executor of one pass.  Asked for, or the library's choice when no LDS tile
+// shape is given: the kernel-split wave executor for a column pass whose plan
+// has a wave kernel (measured fastest on one world: 1024x512 K=3 columns
+// 8.1 us against 10.4 us on the LDS executor), the LDS executor for row
+// passes (1024x512: 10.8 us against 12.9 us kernel-split, 16 us one wave per
+// row pair).  lenia_world_autotune times every executor.  An explicit wave
+// request fixes the waves per workgroup through the thread count.
+static int choose_engine(const FftPlan &p, bool rows, int count, uint32_t engine, uint32_t wantNt,
+                         uint32_t wantT, int numCUs, bool &wave, int &nw, bool &split) {
+    const int L = wave_lanes(p.code);
+    const char *pass = rows ? "row" : "column";
+    if (engine > LENIA_ENGINE_WAVE_SPLIT)
+        return fail(LENIA_ERR_INVALID_ARG, "%s pass: unknown engine %u", pass, engine);
+    const bool autoSplit = engine == LENIA_ENGINE_AUTO && !rows && L > 0 && !wantNt && !wantT;
+    wave = engine == LENIA_ENGINE_WAVE || engine == LENIA_ENGINE_WAVE_SPLIT || autoSplit;
+    split = engine == LENIA_ENGINE_WAVE_SPLIT || autoSplit;
+    if (!wave) return LENIA_OK;
+    if (!L)
+        return fail(LENIA_ERR_UNSUPPORTED, "%s pass: plan 0x%x of length %d has no wave executor", pass,
+                    plan_pack(p), p.n);
+    if (wantNt)
+        return fail(LENIA_ERR_INVALID_ARG, "%s pass: the wave executor derives its tile (got %u per tile)",
+                    pass, wantNt);
+    if (split) {
+        if (wantT && wantT != 64 * kMaxKernels)
+            return fail(LENIA_ERR_UNSUPPORTED, "%s pass: the kernel-split wave executor runs %d threads (got %u)",
+                        pass, 64 * kMaxKernels, wantT);
+        nw = kMaxKernels;
+        return LENIA_OK;
+    }
+    if (wantT && wantT != 64 && wantT != 128 && wantT != 256)
+        return fail(LENIA_ERR_UNSUPPORTED, "%s pass: the wave executor runs 64, 128 or 256 threads (got %u)",
+                    pass, wantT);
+    nw = wantT ? (int)wantT / 64 : wave_default_nw(count, L, numCUs);
+    return LENIA_OK;
+}
+
 static int choose_tiles_fft(lenia_world *w, int numCUs) {
-    (void)numCUs;
     const lenia_world_desc &d = w->desc;
     const int W = d.width, H = d.height, Wh = W / 2 + 1, K = (int)d.nKernels;
-    if (!choose_plan(W, K, d.fftPlanW, w->planW, true) ||
-        !pick_shape(w->planW, true, H / 2, K, d.rowPairsPerTile, d.rowThreads, w->P, w->rowT,
-                    w->rowKpar))
+    if (!choose_plan(W, K, d.fftPlanW, w->planW, true))
+        return fail(LENIA_ERR_UNSUPPORTED, "no row-pass plan fits width %d (plan 0x%x)", W, d.fftPlanW);
+    int rc = choose_engine(w->planW, true, H / 2, d.rowEngine, d.rowPairsPerTile, d.rowThreads, numCUs,
+                           w->rowWave, w->rowNW, w->rowSplit);
+    if (rc) return rc;
+    // the workgroup-LDS shape (the executor of a wave world's reverse pass too)
+    if (!pick_shape(w->planW, true, H / 2, K, w->rowWave ? 0 : d.rowPairsPerTile,
+                    w->rowWave ? 0 : d.rowThreads, w->P, w->rowT, w->rowKpar))
         return fail(LENIA_ERR_UNSUPPORTED,
                     "no row-pass shape fits width %d (plan 0x%x, %u row pairs, %u threads)", W,
                     d.fftPlanW, d.rowPairsPerTile, d.rowThreads);
-    if (!choose_plan(H, K, d.fftPlanH, w->planH, false) ||
-        !pick_shape(w->planH, false, Wh, K, d.colsPerTile, d.colThreads, w->C, w->colT,
-                    w->colKpar))
+    if (!choose_plan(H, K, d.fftPlanH, w->planH, false))
+        return fail(LENIA_ERR_UNSUPPORTED, "no column-pass plan fits height %d (plan 0x%x)", H, d.fftPlanH);
+    rc = choose_engine(w->planH, false, Wh, d.colEngine, d.colsPerTile, d.colThreads, numCUs,
+                       w->colWave, w->colNW, w->colSplit);
+    if (rc) return rc;
+    if (!pick_shape(w->planH, false, Wh, K, w->colWave ? 0 : d.colsPerTile,
+                    w->colWave ? 0 : d.colThreads, w->C, w->colT, w->colKpar))
         return fail(LENIA_ERR_UNSUPPORTED,
                     "no column-pass shape fits height %d (plan 0x%x, %u columns, %u threads)", H,
                     d.fftPlanH, d.colsPerTile, d.colThreads);
@@ -678,8 +730,9 @@ FftWorldDev lenia::fft_record(lenia_world *w) {
     f.W = w->desc.width;
     f.H = w->desc.height;
     f.Wh = f.W / 2 + 1;
-    f.P = w->P;
-    f.C = w->C;
+    const int Lw = wave_lanes(w->planW.code), Lh = wave_lanes(w->planH.code);
+    f.P = w->rowWave ? (w->rowSplit ? 1 : w->rowNW) * (64 / Lw) : w->P;
+    f.C = w->colWave ? (w->colSplit ? 1 : w->colNW) * (64 / Lh) : w->C;
     f.rowTiles = (f.H / 2 + f.P - 1) / f.P;
     f.colTiles = (f.Wh + f.C - 1) / f.C;
     f.planW = w->planW;
@@ -687,8 +740,8 @@ FftWorldDev lenia::fft_record(lenia_world *w) {
     f.K = (int)w->desc.nKernels;
     f.mP = div_magic(f.P);
     f.mW = div_magic(f.W);
-    f.rowThreads = w->rowT;
-    f.colThreads = w->colT;
+    f.rowThreads = w->rowWave ? 64 * w->rowNW : w->rowT;
+    f.colThreads = w->colWave ? 64 * w->colNW : w->colT;
     f.g = w->g;
     f.twW = w->twW;
     f.twH = w->twH;
@@ -700,8 +753,15 @@ FftWorldDev lenia::fft_record(lenia_world *w) {
     w->fdev = f;
     w->nRowTiles = f.rowTiles;
     w->nColTiles = f.colTiles;
-    w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
-    w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
+    if (w->rowWave)
+        w->ldsRows = w->rowSplit ? std::max(wave_split_lds_bytes(f.W, Lw, true), wave_lds_bytes(f.W, Lw, 1))
+                                 : wave_lds_bytes(f.W, Lw, w->rowNW);
+    else
+        w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
+    if (w->colWave)
+        w->ldsCols = w->colSplit ? wave_split_lds_bytes(f.H, Lh, false) : wave_lds_bytes(f.H, Lh, w->colNW);
+    else
+        w->ldsCols = lds_cols(f.H, f.C, f.K, w->colKpar);
     return f;
 }
 
@@ -972,12 +1032,37 @@ int lenia::world_apply_init(lenia_world *w) {
     return LENIA_OK;
 }
 
+hipError_t lenia::world_launch_rows(lenia_world *w, int mode, const FftWorldDev *rec, uint32_t off,
+                                    int kSel, float *out, bool extras, hipStream_t s) {
+    extras = extras && mode == 1;
+    if (w->rowWave) {
+        const int vc = wave_class(w->planW.code, w->planW.n);
+        if (w->rowSplit && mode == 1)
+            return launch_wave_rows_split(extras, vc, rec, w->rowTiles, w->nRowTiles, w->ldsRows, off, s);
+        return launch_wave_rows(mode, w->rowSplit ? 1 : w->rowNW, vc, extras, rec, w->rowTiles,
+                                w->nRowTiles, w->ldsRows, off, kSel, out, s);
+    }
+    // forward-only passes run the serial form, potentials the side-by-side one
+    const bool kpar = mode == 1 ? w->rowKpar : mode == 2;
+    return launch_fft_rows(mode, w->rowT, extras, kpar, rec, w->rowTiles, w->nRowTiles, w->nRowTiles,
+                           w->ldsRows, off, kSel, out, s, mode == 1 ? w->row_tile_cells() : 0);
+}
+
+hipError_t lenia::world_launch_cols(lenia_world *w, const FftWorldDev *rec, hipStream_t s) {
+    if (w->colWave) {
+        const int vc = wave_class(w->planH.code, w->planH.n);
+        if (w->colSplit) return launch_wave_cols_split(vc, rec, w->colTiles, w->nColTiles, w->ldsCols, s);
+        return launch_wave_cols(w->colNW, vc, rec, w->colTiles, w->nColTiles, w->ldsCols, s);
+    }
+    return launch_fft_cols(w->colT, w->colKpar, rec, w->colTiles, w->nColTiles, w->nColTiles,
+                           w->ldsCols, s, w->col_tile_cells());
+}
+
 // F must hold the spectrum of the current state before a row+column step
 int lenia::world_ensure_spectrum(lenia_world *w) {
     if (!w->isFft || w->specValid) return LENIA_OK;
-    lenia_ctx *ctx = w->ctx;
-    hipError_t e = launch_fft_rows(0, w->rowT, false, false, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                   w->nRowTiles, w->ldsRows, 0, 0, nullptr, ctx->stream);
+    hipError_t e = world_launch_rows(w, 0, (const FftWorldDev *)w->devDesc, 0, 0, nullptr, false,
+                                     w->ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "forward row pass");
     w->specValid = true;
     return LENIA_OK;
@@ -993,15 +1078,11 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
     hipError_t e;
     if (w->isFft) {
         if (ev) launch_events() = {ev[0], ev[1]};
-        e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles,
-                            w->nColTiles, w->nColTiles, w->ldsCols, ctx->stream,
-                            w->col_tile_cells());
+        e = world_launch_cols(w, (const FftWorldDev *)w->devDesc, ctx->stream);
         if (ev) launch_events() = {ev[2], ev[3]};
         if (e == hipSuccess)
-            e = launch_fft_rows(1, w->rowT, w->extras(), w->rowKpar,
-                                (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream,
-                                w->row_tile_cells());
+            e = world_launch_rows(w, 1, (const FftWorldDev *)w->devDesc, off, 0, nullptr, w->extras(),
+                                  ctx->stream);
     } else if (w->isMfma) {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
@@ -1154,7 +1235,10 @@ extern "C" int lenia_world_set_params(lenia_world *w, const lenia_world_desc *de
     uint32_t keepPW = w->desc.fftPlanW, keepPH = w->desc.fftPlanH;
     uint32_t keepP = w->desc.rowPairsPerTile, keepC = w->desc.colsPerTile;
     uint32_t keepRT = w->desc.rowThreads, keepCT = w->desc.colThreads;
+    uint32_t keepRE = w->desc.rowEngine, keepCE = w->desc.colEngine;
     w->desc = *desc;
+    w->desc.rowEngine = keepRE;
+    w->desc.colEngine = keepCE;
     w->desc.fftPlanW = keepPW;
     w->desc.fftPlanH = keepPH;
     w->desc.rowPairsPerTile = keepP;
@@ -1216,12 +1300,10 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
     if (w->isFft) {
         int rc = world_ensure_spectrum(w);
         if (rc) return rc;
-        e = launch_fft_cols(w->colT, w->colKpar, (const FftWorldDev *)w->devDesc, w->colTiles, w->nColTiles, w->nColTiles,
-                            w->ldsCols, ctx->stream);
+        e = world_launch_cols(w, (const FftWorldDev *)w->devDesc, ctx->stream);
         if (e == hipSuccess)
-            e = launch_fft_rows(2, w->rowT, false, true, (const FftWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                                w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut,
-                                ctx->stream);
+            e = world_launch_rows(w, 2, (const FftWorldDev *)w->devDesc, 0, (int)k, w->scratchOut, false,
+                                  ctx->stream);
     } else if (w->isMfma) {
         e = launch_mfma(1, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
                         w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
@@ -1246,6 +1328,16 @@ extern "C" int lenia_world_get_desc(lenia_world *w, lenia_world_desc *out) {
         out->colsPerTile = w->C;
         out->rowThreads = w->rowT;
         out->colThreads = w->colT;
+        out->rowEngine = w->rowWave ? (w->rowSplit ? LENIA_ENGINE_WAVE_SPLIT : LENIA_ENGINE_WAVE) : LENIA_ENGINE_LDS;
+        out->colEngine = w->colWave ? (w->colSplit ? LENIA_ENGINE_WAVE_SPLIT : LENIA_ENGINE_WAVE) : LENIA_ENGINE_LDS;
+        if (w->rowWave) {
+            out->rowPairsPerTile = 0;
+            out->rowThreads = 64 * w->rowNW;
+        }
+        if (w->colWave) {
+            out->colsPerTile = 0;
+            out->colThreads = 64 * w->colNW;
+        }
     }
     return LENIA_OK;
 }

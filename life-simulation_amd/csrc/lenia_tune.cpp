@@ -35,15 +35,22 @@ namespace {
 struct Shape {
     int nt, T;
     bool kpar;
+    bool wave;   // wave executor with T / 64 waves per workgroup (nt, kpar unused)
+    bool split;  // kernel-split wave executor
 };
 
-// the candidate shapes of one pass that the plan, registers and LDS allow
+// the candidate shapes of one pass that the plan, registers and LDS allow,
+// and the wave executor at 1, 2 and 4 waves per workgroup when the plan has one
 std::vector<Shape> candidates(const FftPlan &p, bool rows, int count, int K) {
     std::vector<Shape> out;
     for (int kp = 1; kp >= 0; --kp)
         for (int nt = 1; nt <= 8 && nt <= count; nt *= 2)
             for (int T = 64; T <= 1024; T *= 2)
-                if (shape_ok(p, rows, nt, K, T, kp != 0)) out.push_back({nt, T, kp != 0});
+                if (shape_ok(p, rows, nt, K, T, kp != 0)) out.push_back({nt, T, kp != 0, false, false});
+    if (wave_lanes(p.code)) {
+        for (int nw = 1; nw <= 4; nw *= 2) out.push_back({0, 64 * nw, false, true, false});
+        out.push_back({0, 64 * kMaxKernels, false, true, true});
+    }
     return out;
 }
 
@@ -89,11 +96,34 @@ extern "C" int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msP
     const int K = (int)w->desc.nKernels;
     const int H = w->desc.height, Wh = w->desc.width / 2 + 1;
     double best[2] = {1e30, 1e30};
-    Shape bestRow{w->P, w->rowT, w->rowKpar}, bestCol{w->C, w->colT, w->colKpar};
+    const Shape keepRow{w->P, w->rowT, w->rowKpar, false, false}, keepCol{w->C, w->colT, w->colKpar, false, false};
+    Shape bestRow = w->rowWave ? Shape{0, 64 * w->rowNW, false, true, w->rowSplit} : keepRow;
+    Shape bestCol = w->colWave ? Shape{0, 64 * w->colNW, false, true, w->colSplit} : keepCol;
+    // a wave candidate keeps the LDS shape fields as they are
+    auto set_row = [&](const Shape &s) {
+        w->rowWave = s.wave;
+        w->rowSplit = s.split;
+        if (s.wave) {
+            w->rowNW = s.T / 64;
+        } else {
+            w->P = s.nt;
+            w->rowT = s.T;
+            w->rowKpar = s.kpar;
+        }
+    };
+    auto set_col = [&](const Shape &s) {
+        w->colWave = s.wave;
+        w->colSplit = s.split;
+        if (s.wave) {
+            w->colNW = s.T / 64;
+        } else {
+            w->C = s.nt;
+            w->colT = s.T;
+            w->colKpar = s.kpar;
+        }
+    };
     for (const Shape &s : candidates(w->planW, true, H / 2, K)) {
-        w->P = s.nt;
-        w->rowT = s.T;
-        w->rowKpar = s.kpar;
+        set_row(s);
         double ms[2];
         rc = time_shape(w, nSteps, ms);
         if (rc) break;
@@ -102,14 +132,10 @@ extern "C" int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msP
             bestRow = s;
         }
     }
-    w->P = bestRow.nt;
-    w->rowT = bestRow.T;
-    w->rowKpar = bestRow.kpar;
+    set_row(bestRow);
     for (const Shape &s : candidates(w->planH, false, Wh, K)) {
         if (rc) break;
-        w->C = s.nt;
-        w->colT = s.T;
-        w->colKpar = s.kpar;
+        set_col(s);
         double ms[2];
         rc = time_shape(w, nSteps, ms);
         if (rc) break;
@@ -118,14 +144,14 @@ extern "C" int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msP
             bestCol = s;
         }
     }
-    w->C = bestCol.nt;
-    w->colT = bestCol.T;
-    w->colKpar = bestCol.kpar;
+    set_col(bestCol);
     w->shapeVersion += 1;
-    w->desc.rowPairsPerTile = w->P;
-    w->desc.colsPerTile = w->C;
-    w->desc.rowThreads = w->rowT;
-    w->desc.colThreads = w->colT;
+    w->desc.rowEngine = w->rowWave ? (w->rowSplit ? LENIA_ENGINE_WAVE_SPLIT : LENIA_ENGINE_WAVE) : LENIA_ENGINE_LDS;
+    w->desc.colEngine = w->colWave ? (w->colSplit ? LENIA_ENGINE_WAVE_SPLIT : LENIA_ENGINE_WAVE) : LENIA_ENGINE_LDS;
+    w->desc.rowPairsPerTile = w->rowWave ? 0 : w->P;
+    w->desc.colsPerTile = w->colWave ? 0 : w->C;
+    w->desc.rowThreads = w->rowWave ? 64 * w->rowNW : w->rowT;
+    w->desc.colThreads = w->colWave ? 64 * w->colNW : w->colT;
     const int rc2 = world_upload_desc(w);
     // restore the state (its spectrum is rebuilt by the next step) and clock
     e = hipMemcpyAsync(w->current_state(), w->scratchOut, cells * sizeof(float),
@@ -239,6 +265,7 @@ bool time_plans(lenia_ctx *ctx, lenia_world_desc d, uint32_t planW, uint32_t pla
     d.fftPlanW = planW;
     d.fftPlanH = planH;
     d.rowPairsPerTile = d.colsPerTile = d.rowThreads = d.colThreads = 0;
+    d.rowEngine = d.colEngine = LENIA_ENGINE_AUTO;
     lenia_world *w = nullptr;
     if (lenia_world_create(ctx, &d, &w) != LENIA_OK) return false;
     int rc = lenia_world_reset(w, 1u);

@@ -188,7 +188,8 @@ class World:
         check(_abi.lib().lenia_world_get_desc(self._h, C.byref(d)))
         return {"fftPlanW": d.fftPlanW, "fftPlanH": d.fftPlanH,
                 "rowPairsPerTile": d.rowPairsPerTile, "colsPerTile": d.colsPerTile,
-                "rowThreads": d.rowThreads, "colThreads": d.colThreads}
+                "rowThreads": d.rowThreads, "colThreads": d.colThreads,
+                "rowEngine": d.rowEngine, "colEngine": d.colEngine}
 
     def close(self):
         if self._h and getattr(self, "_owned", True):
@@ -244,6 +245,16 @@ class Batch:
         c, r = C.c_uint32(), C.c_uint32()
         check(_abi.lib().lenia_batch_get_grids(self._h, C.byref(c), C.byref(r)))
         return c.value, r.value
+
+    def set_waves(self, row_waves: int, col_waves: int):
+        """waves per workgroup of the wave executor per pass (0: every world on
+        the workgroup-LDS executor); same bits either way"""
+        check(_abi.lib().lenia_batch_set_waves(self._h, row_waves, col_waves))
+
+    def waves(self) -> tuple[int, int]:
+        r, c = C.c_uint32(), C.c_uint32()
+        check(_abi.lib().lenia_batch_get_waves(self._h, C.byref(r), C.byref(c)))
+        return r.value, c.value
 
     def close(self):
         if self._h:

@@ -35,7 +35,7 @@ EXPORTS = [
     "lenia_world_write_state", "lenia_world_state_device", "lenia_world_potential",
     "lenia_world_get_desc", "lenia_world_pass_times",
     "lenia_batch_create", "lenia_batch_destroy", "lenia_batch_step",
-    "lenia_batch_set_grids", "lenia_batch_get_grids",
+    "lenia_batch_set_grids", "lenia_batch_get_grids", "lenia_batch_set_waves", "lenia_batch_get_waves",
     "lenia_rollout_forward", "lenia_rollout_backward",
     "lenia_world_autotune", "lenia_batch_autotune", "lenia_world_set_organism",
     "lenia_world_save_npy", "lenia_world_load_npy",
@@ -92,6 +92,8 @@ class WorldDesc(C.Structure):
         ("colsPerTile", C.c_uint32),
         ("rowThreads", C.c_uint32),
         ("colThreads", C.c_uint32),
+        ("rowEngine", C.c_uint32),
+        ("colEngine", C.c_uint32),
     ]
 
 
@@ -185,6 +187,8 @@ def lib():
         "lenia_batch_autotune": (i32, [vp, u32, C.POINTER(C.c_double)]),
         "lenia_batch_set_grids": (i32, [vp, u32, u32]),
         "lenia_batch_get_grids": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
+        "lenia_batch_set_waves": (i32, [vp, u32, u32]),
+        "lenia_batch_get_waves": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
         "lenia_rollout_backward": (i32, [vp, u32, u32, vp, vp, vp, C.POINTER(ParamGrads), i32]),
         "lenia_world_set_organism": (i32, [vp, fp, u32, u32, u32, u32]),
         "lenia_world_save_npy": (i32, [vp, C.c_char_p]),
@@ -311,4 +315,6 @@ def world_desc(w: dict) -> WorldDesc:
     d.colsPerTile = int(w.get("colsPerTile", 0))
     d.rowThreads = int(w.get("rowThreads", 0))
     d.colThreads = int(w.get("colThreads", 0))
+    d.rowEngine = int(w.get("rowEngine", 0))
+    d.colEngine = int(w.get("colEngine", 0))
     return d

@@ -52,6 +52,8 @@ static void layouts(void) {
     OFF(lenia_world_desc, colsPerTile);
     OFF(lenia_world_desc, rowThreads);
     OFF(lenia_world_desc, colThreads);
+    OFF(lenia_world_desc, rowEngine);
+    OFF(lenia_world_desc, colEngine);
     SZ(lenia_param_grads);
     OFF(lenia_param_grads, dMu);
     OFF(lenia_param_grads, dSigma);

@@ -33,7 +33,7 @@ def test_struct_layouts():
     # lenia_kernel_desc: 6 scalars + betas[4] + 3 scalars, all 4 bytes
     assert C.sizeof(_abi.KernelDesc) == 4 * 13
     # lenia_world_desc: 11 leading scalars, 4 kernels, 7 init fields, 6 tuning fields
-    assert C.sizeof(_abi.WorldDesc) == 4 * 11 + 4 * 52 + 4 * 7 + 4 * 6
+    assert C.sizeof(_abi.WorldDesc) == 4 * 11 + 4 * 52 + 4 * 7 + 4 * 6 + 4 * 2
 
 
 def test_version_and_last_error():

@@ -72,6 +72,13 @@ def test_fft_batch_bit_exact_vs_alone(ctx, graph_steps):
               f"mass {float(want.sum()):.3f}")
 
 
+QUEUES_WITHDRAWN = ("tile-queue grids withdrawn: this test aborted on the GPU (2026-12-04) once the "
+                    "column pass ran several launches side by side; the cause was not found, and the "
+                    "queues, 3x slower than one workgroup per tile, were removed (lenia_batch_set_grids "
+                    "now refuses non-zero grids)")
+
+
+@pytest.mark.skip(reason=QUEUES_WITHDRAWN)
 @pytest.mark.parametrize("grids", [[(1, 1)], [(2, 4), (1, 2)], [(4, 1), (0, 0), (1, 0)]])
 def test_fft_batch_tile_queues_bit_exact(ctx, grids):
     """resident grids taking tiles from the queues (a few workgroups per CU,
@@ -85,6 +92,7 @@ def test_fft_batch_tile_queues_bit_exact(ctx, grids):
         assert got[i].tobytes() == want.tobytes(), (grids, i, d["width"], d["height"])
 
 
+@pytest.mark.skip(reason=QUEUES_WITHDRAWN)
 def test_fft_batch_tile_queues_many_worlds(ctx):
     """the 256-world sweep with one resident workgroup per CU per pass (more
     than ten tiles per workgroup), two graph replays"""
@@ -163,10 +171,12 @@ def test_batch_errors(ctx):
             with pytest.raises(lenia.LeniaError):
                 bt.step(10, 4)  # not a multiple of the graph length
             with pytest.raises(lenia.LeniaError):
-                bt.set_grids(65, 0)  # more than 64 workgroups per CU
+                bt.set_grids(65, 0)  # tile-queue grids are withdrawn: non-zero grids refused
         with lenia.Batch(ctx, [b]) as bt:
             with pytest.raises(lenia.LeniaError):
-                bt.set_grids(1, 1)  # tile queues are for the FFT form
+                bt.set_grids(1, 1)  # refused for every form
+            bt.set_grids(0, 0)  # one workgroup per tile: accepted
+            assert bt.grids() == (0, 0)
     finally:
         a.close()
         b.close()
@@ -216,7 +226,7 @@ def test_sweep_256_as_benchmarked(ctx):
             w.frame(0, 1)
         with lenia.Batch(ctx, worlds) as bt:
             ms = bt.autotune(10)
-            print(f"sweep 256: tuned grids (columns, rows per CU) {bt.grids()}")
+            print(f"sweep 256: tuned executors (row, column waves per workgroup) {bt.waves()}")
             bt.step(50, 50)
             bt.step(200, 50)
             got = [w.read() for w in worlds]
