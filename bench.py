@@ -178,13 +178,14 @@ def tune_sweep_plans(ctx) -> dict:
 
 
 def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int = 64,
-                tune_plans: bool = True) -> dict:
+                tune_plans: bool = True, waves=None) -> dict:
     """256 uneven worlds stepped in one batch (graph-replayed) vs the same worlds
     stepped one world per launch sequence, one after another (the PresetLab
     loop, Apps/LifePresetLab/main.cpp:718-806).  The plain loop replays each
     world's own 32-step graphs (plain_steps is a multiple of 32).  Both sides
     are timed the same way, by the host clock between two synchronisations;
-    the batch's GPU-event time is reported too."""
+    the batch's GPU-event time is reported too.  waves: fixed (row, column)
+    executors (lenia_batch_set_waves) instead of the autotuner's choice."""
     import lenia
     from lenia_cases import sweep_worlds
 
@@ -205,6 +206,8 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
                          fft_bytes_per_cell(d["width"], len(d["kernels"]))["step"] for d in descs)
         with lenia.Batch(ctx, worlds) as b:
             tuned_ms = b.autotune(10)  # launch shapes for this batch (bits unchanged)
+            if waves is not None:
+                b.set_waves(*waves)
             b.step(graph_steps, graph_steps)  # warm-up + graph capture
             ctx.sync()
             t0 = time.perf_counter()
@@ -514,6 +517,12 @@ def main() -> int:
     ap.add_argument("--no-live", action="store_true", help="skip the live-field rate")
     ap.add_argument("--no-plan-tune", action="store_true",
                     help="keep the built-in radix plans (no lenia_plan_autotune)")
+    ap.add_argument("--engines", default="", metavar="ROW,COL",
+                    help="pass executors of the headline world: lds, wave or split (with --tiles: "
+                         "a wave pass takes its threads from ROWT / COLT and needs ROWP / COLC 0)")
+    ap.add_argument("--plans", default="", metavar="PLANW,PLANH",
+                    help="radix plans of the headline world (hex, as in the descriptor); implies "
+                         "no plan tuning")
     ap.add_argument("--tiles", default="", metavar="ROWT,ROWP,COLT,COLC",
                     help="fixed launch shapes of the headline world instead of autotuning "
                          "(row threads, row pairs per tile, column threads, columns per tile)")
@@ -522,6 +531,9 @@ def main() -> int:
     ap.add_argument("--only", choices=["sweep", "adjoint", "direct", "matrix", "lifebench", "live"],
                     default="", help="run one side leg alone and print its JSON object (profiling)")
     ap.add_argument("--sweep-steps", type=int, default=1000, help="timed batched sweep steps")
+    ap.add_argument("--sweep-waves", default="", metavar="ROW,COL",
+                    help="fixed executors of the sweep batch (waves per workgroup, 0 = LDS, "
+                         "3 = kernel-split) instead of the autotuner's choice")
     ap.add_argument("--plain-steps", type=int, default=64,
                     help="steps per world of the one-world-per-launch loop, a multiple of 32 "
                          "(0: skip it)")
@@ -537,7 +549,8 @@ def main() -> int:
     cpu_s = 0.0 if args.no_cpu else max(1.0, args.cpu_seconds / 4)
     if args.only:
         import lenia
-        legs = {"sweep": lambda c: sweep_bench(c, args.sweep_steps, plain_steps=args.plain_steps,
+        sw = tuple(int(x) for x in args.sweep_waves.split(",")) if args.sweep_waves else None
+        legs = {"sweep": lambda c: sweep_bench(c, args.sweep_steps, plain_steps=args.plain_steps, waves=sw,
                                                tune_plans=not args.no_plan_tune),
                 "adjoint": adjoint_bench, "direct": direct_forms_bench,
                 "matrix": lambda c: matrix_bench(c, cpu_s, not args.no_plan_tune),
@@ -554,16 +567,24 @@ def main() -> int:
     if args.tiles:
         rt, rp, ct, cc = (int(x) for x in args.tiles.split(","))
         desc.update(rowThreads=rt, rowPairsPerTile=rp, colThreads=ct, colsPerTile=cc)
+    if args.engines:
+        codes = dict({v: k for k, v in ENGINE_NAMES.items()}, split=ENGINE_WAVE_SPLIT)
+        re_, ce = args.engines.split(",")
+        desc.update(rowEngine=codes[re_], colEngine=codes[ce])
+    if args.plans:
+        pw, ph = (int(x, 16) for x in args.plans.split(","))
+        desc.update(fftPlanW=pw, fftPlanH=ph)
+        args.no_plan_tune = True
     W, H, K = desc["width"], desc["height"], len(desc["kernels"])
     cells = W * H
     with lenia.Context(0) as ctx:
-        if not args.tiles and not args.no_plan_tune:
+        if not args.tiles and not args.no_plan_tune and not args.plans:
             # radix plans of the headline world's width and height (plan cache)
             lenia.plan_autotune(ctx, desc, 10)
         w = lenia.World(ctx, desc)
         w.reset(7777)
         w.frame(0, 1)
-        if not args.tiles:
+        if not (args.tiles or args.engines):
             w.autotune(20)  # launch shapes for this world (bits unchanged)
         w.step(args.warmup)
         ctx.sync()
@@ -578,9 +599,10 @@ def main() -> int:
         A = w.read()
         w.close()
         live = None if args.no_live else live_field_bench(ctx, args.steps)
+        sw = tuple(int(x) for x in args.sweep_waves.split(",")) if args.sweep_waves else None
         batched = None if args.no_sweep else sweep_bench(ctx, args.sweep_steps,
                                                           plain_steps=args.plain_steps,
-                                                          tune_plans=not args.no_plan_tune)
+                                                          tune_plans=not args.no_plan_tune, waves=sw)
         adjoint = None if args.no_adjoint else adjoint_bench(ctx)
         direct = None if args.no_direct else direct_forms_bench(ctx)
         matrix = None if args.no_matrix else matrix_bench(ctx, cpu_s, not args.no_plan_tune)
@@ -612,7 +634,7 @@ def main() -> int:
                    "colThreads": tiles["colThreads"],
                    "rowEngine": ENGINE_NAMES[tiles["rowEngine"]],
                    "colEngine": ENGINE_NAMES[tiles["colEngine"]],
-                   "autotuned": not args.tiles,
+                   "autotuned": not (args.tiles or args.engines),
                    "radix_plans_tuned": not (args.tiles or args.no_plan_tune)},
         "passes": roof,
         "pass_samples": PASS_SAMPLES,

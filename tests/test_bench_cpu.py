@@ -34,7 +34,7 @@ def test_traffic_for_either_dominant_pass():
     sys.path.insert(0, ROOT)
     import bench
 
-    for k in ("k_fft_rows", "k_fft_cols"):
+    for k in ("k_fft_rows", "k_wave_cols_split"):
         t = bench.measured_traffic(k)
         assert isinstance(t, int) and 10_000_000 < t < 20_000_000
     assert bench.measured_traffic("k_unknown") is None
