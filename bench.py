@@ -257,11 +257,13 @@ def sweep_bench(ctx, steps: int = 1000, graph_steps: int = 50, plain_steps: int 
             w.close()
 
 
-def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
+def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 20) -> dict:
     """differentiable-rollout world (256x256, two ring kernels R=16): wall time
     of the T-step forward rollout (which keeps checkpoints every `ckpt` steps)
     and of its reverse pass (segment recompute included), host arrays in and
-    out, best of reps; also the reverse pass that reruns the forward sweep"""
+    out, best of reps after a second of warm-up calls (the clocks of a GPU that
+    was idle ramp up over the first calls); also the reverse pass that reruns
+    the forward sweep"""
     import numpy as np
 
     import lenia
@@ -279,6 +281,10 @@ def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
         w.autotune(10)
         w.rollout(A0, T, ckpt)
         w.rollout_grad(None, T, gT, ckpt)  # workspace, graphs, spectra
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 1.0:
+            w.rollout(A0, T, ckpt)
+            w.rollout_grad(None, T, gT, ckpt)
         fwd, bwd, full = [], [], []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -295,6 +301,8 @@ def adjoint_bench(ctx, T: int = 64, ckpt: int = 8, reps: int = 5) -> dict:
     return {"world": "256x256 K=2 R=16 FFT", "T": T, "ckpt_every": ckpt,
             "forward_ms": round(f, 4), "backward_ms": round(b, 4),
             "forward_plus_adjoint_over_forward": round((f + b) / f, 3),
+            "median_forward_ms": round(float(np.median(fwd)) * 1e3, 4),
+            "median_backward_ms": round(float(np.median(bwd)) * 1e3, 4),
             "backward_with_forward_sweep_ms": round(fb, 4),
             "mcell_steps_per_s_forward": round(256 * 256 * T / f / 1e3, 1)}
 

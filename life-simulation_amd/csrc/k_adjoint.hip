@@ -17,14 +17,17 @@
 //
 // The recompute of a segment runs the normal step kernels with their save
 // pointers set (FftWorldDev::saveA/saveU/saveAhat), so each step leaves its
-// input state A_t, its potentials u_k and the 2D spectrum Ahat_t.  One reverse
-// step is then three launches (row transforms along x on row pairs packed as
-// one complex row, column transforms along y on the half spectrum):
-//   k_adj_mid        per cell gp, gu_k and the parameter partials from the
+// input state A_t, its potentials u_k and the 2D spectrum Ahat_t.  A reverse
+// step is a row launch and a column launch (row transforms along x on row
+// pairs packed as one complex row, column transforms along y on the half
+// spectrum):
+//   k_adj_mid<EDGE>  EDGE: first finish the later step, g_t+1 = gp + row
+//                    inverse of G1 (kept in registers, never stored); then
+//                    per cell gp, gu_k and the parameter partials from the
 //                    saved u_k and A_t; row forward of gu_k -> FG_k
 //   k_adj_cols       column forward of FG_k; shell partials against Ahat_t;
 //                    G1 = column inverse of sum_k S_k FG_k^
-//   k_adj_edge       g_t = gp + row inverse of G1
+//   k_adj_edge       after the first step only: g_0 = gp + row inverse of G1
 // The kernels reuse the LDS FFT of the forward path (same plans, same
 // butterflies).  Each workgroup adds its partial sums to its own FP64 slots
 // (one row pair or one spectrum column per workgroup, every step in order),
@@ -134,10 +137,14 @@ __device__ __forceinline__ void block_sums(const float (&v)[NV], int n, float sc
     }
 }
 
-template <int T>
+// EDGE: the incoming gradient of this step is gpb + row inverse of G1 (the
+// later step's edge, fused here); gpb is then overwritten in place with this
+// step's gp (each cell is read and written by the same thread).  Otherwise
+// the incoming gradient is read from g.
+template <int T, bool EDGE>
 __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const float *A,
-                                               const float *g, float *gpb, float *FG,
-                                               double *accRows) {
+                                               const float *g, const float *G1, float *gpb,
+                                               float *FG, double *accRows) {
     extern __shared__ float2 smem[];
     const int tid = threadIdx.x;
     const int W = a.W, H = a.H, Wh = W / 2 + 1, H2 = H / 2, K = a.K;
@@ -146,7 +153,38 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
     float2 *tw = smem;
     float2 *Z = smem + W;
     float *red = reinterpret_cast<float *>(Z + lds_span(K * W));
+    const LENIA_GLOBAL float *Ag = gp(A);
+    const LENIA_GLOBAL float *gg = gp(g);
+    const LENIA_GLOBAL float *Ug = gp(U);
+    LENIA_GLOBAL float *gpo = gp(gpb);
+    const size_t n = (size_t)W * H;
+    // a cell pair's operands (rows 2rp and 2rp+1 at column x): potentials,
+    // state, incoming gradient (EDGE: the gp part of it)
+    auto load_x = [&](int x, float2 (&u2)[4], float2 &a2, float2 &g2) {
+        const size_t i0 = (size_t)(2 * rp) * W + x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            u2[k] = k < K ? make_float2(Ug[k * n + i0], Ug[k * n + i0 + W]) : make_float2(0.f, 0.f);
+        a2 = make_float2(Ag[i0], Ag[i0 + W]);
+        g2 = EDGE ? make_float2(gpo[i0], gpo[i0 + W]) : make_float2(gg[i0], gg[i0 + W]);
+    };
+    // the operands of each thread's first kPre columns are requested before
+    // the transforms, which hide their latency
+    constexpr int kPre = 2;
+    float2 pu[kPre][4], pa[kPre], pg[kPre];
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int x = tid + i * T;
+        if (x < W) load_x(x, pu[i], pa[i], pg[i]);
+    }
     load_twiddles(tw, a.twW, W, tid, T);
+    if (EDGE) {
+        // slot 0 holds the later step's row inverse; the K slots written
+        // below reuse it only at each thread's own cells, after reading them
+        gather_row_pair<false>(Z, 0, G1, W, H2, rp, tid, T);
+        __syncthreads();
+        fft_run<true, kAV, T>(Z, a.planW, 1, tw, tid);
+    }
     const GrowthParams &gr = *a.gDev;
     const float dt = gr.dt;
     float mu[4], bs[4], wk[4], sg[4];
@@ -160,18 +198,15 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
     float vals[13];
 #pragma unroll
     for (int v = 0; v < 13; ++v) vals[v] = 0.0f;
-    const LENIA_GLOBAL float *Ag = gp(A);
-    const LENIA_GLOBAL float *gg = gp(g);
-    const LENIA_GLOBAL float *Ug = gp(U);
-    LENIA_GLOBAL float *gpo = gp(gpb);
-    const size_t n = (size_t)W * H;
-    for (int x = tid; x < W; x += T) {
-        float2 u2[4], gu2[4];
-        const size_t i0 = (size_t)(2 * rp) * W + x;
+    auto cell = [&](int x, const float2 (&u2)[4], float2 a2, float2 g2) {
+        float2 gu2[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            u2[k] = k < K ? make_float2(Ug[k * n + i0], Ug[k * n + i0 + W]) : make_float2(0.f, 0.f);
-            gu2[k] = make_float2(0.f, 0.f);
+        for (int k = 0; k < 4; ++k) gu2[k] = make_float2(0.f, 0.f);
+        float gin[2] = {g2.x, g2.y};
+        if (EDGE) {
+            const float2 z = Z[sw(x)];
+            gin[0] = g2.x + z.x;
+            gin[1] = g2.y + z.y;
         }
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
@@ -183,8 +218,8 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
                 b[k] = bell_of(u[k], mu[k], bs[k]);
                 if (k < K) sumStd += wk[k] * (2.0f * b[k] - 1.0f);
             }
-            const float pre = Ag[i] + dt * sumStd;
-            const float gpv = (pre > 0.0f && pre < 1.0f) ? gg[i] : 0.0f;
+            const float pre = (half ? a2.y : a2.x) + dt * sumStd;
+            const float gpv = (pre > 0.0f && pre < 1.0f) ? gin[half] : 0.0f;
             gpo[i] = gpv;
             vals[0] += gpv * sumStd;
 #pragma unroll
@@ -204,6 +239,16 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (k < K) Z[sw(k * W + x)] = gu2[k];
+    };
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int x = tid + i * T;
+        if (x < W) cell(x, pu[i], pa[i], pg[i]);
+    }
+    for (int x = tid + kPre * T; x < W; x += T) {
+        float2 u2[4], a2, g2;
+        load_x(x, u2, a2, g2);
+        cell(x, u2, a2, g2);
     }
     __syncthreads();
     fft_run<false, kAV, T>(Z, a.planW, K, tw, tid);
@@ -226,6 +271,30 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
     const size_t sStride = (size_t)Wh * H;
     float2 *tw = smem;
     float2 *X = smem + H;  // K slots
+    const int nb = a.nShells;  // total (k, j) pairs, <= 16
+    const LENIA_GLOBAL float2 *Ah = gp(reinterpret_cast<const float2 *>(Ahat)) + (size_t)c * H;
+    const LENIA_GLOBAL float *Dc = gp(a.D) + (size_t)c * H;
+    const LENIA_GLOBAL float *Sc = gp(a.S) + (size_t)c * H;
+    // per spectrum row ky: the state spectrum, the shell-gradient spectra and
+    // the kernel spectra at (c, ky)
+    auto load_y = [&](int ky, float2 &ah, float (&d)[16], float (&sk)[4]) {
+        ah = gld2(Ah + ky);
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            if (i < nb) d[i] = Dc[(size_t)i * sStride + ky];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < K) sk[k] = Sc[(size_t)k * sStride + ky];
+    };
+    // each thread's first kPre rows are requested before the transform
+    constexpr int kPre = 2;
+    float2 pah[kPre];
+    float pd[kPre][16], ps[kPre][4];
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int ky = tid + i * T;
+        if (ky < H) load_y(ky, pah[i], pd[i], ps[i]);
+    }
     load_twiddles(tw, a.twH, H, tid, T);
     for (int e = tid; e < K * H; e += T) {
         int s = e / H, ky = e - s * H;
@@ -234,34 +303,43 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
     __syncthreads();
     fft_run<false, kAV, T>(X, a.planH, K, tw, tid);
     float2 *Y = X + lds_span(K * H);
-    // shell partial sums: thread-strided over ky, then a fixed-order LDS tree
+    // shell partial sums: thread-strided over ky, then a fixed-order
+    // workgroup sum; Y = sum_k S_k * gu_k^
     const float wt = (c == 0 || c == Wh - 1) ? 1.0f : 2.0f;
-    const int nb = a.nShells;  // total (k, j) pairs, <= 16
     float acc[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
-    for (int ky = tid; ky < H; ky += T) {
-        float2 ah = gld2(gp(reinterpret_cast<const float2 *>(Ahat)) + (size_t)c * H + ky);
+    auto row = [&](int ky, float2 ah, const float (&d)[16], const float (&sk)[4]) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             if (i < nb) {
                 int k = a.shellK[i];
                 float2 gu = X[sw(k * H + ky)];
                 float re = gu.x * ah.x + gu.y * ah.y;  // Re(conj(gu) * A)
-                acc[i] += gp(a.D)[(size_t)i * sStride + (size_t)c * H + ky] * re;
+                acc[i] += d[i] * re;
             }
         }
-    }
-    // Y = sum_k S_k * gu_k^
-    for (int e = tid; e < H; e += T) {
         float2 y = make_float2(0.f, 0.f);
-        for (int k = 0; k < K; ++k) {
-            float s = gp(a.S)[k * sStride + (size_t)c * H + e];
-            float2 x = X[sw(k * H + e)];
-            y.x = fmaf(s, x.x, y.x);
-            y.y = fmaf(s, x.y, y.y);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < K) {
+                float2 x = X[sw(k * H + ky)];
+                y.x = fmaf(sk[k], x.x, y.x);
+                y.y = fmaf(sk[k], x.y, y.y);
+            }
         }
-        Y[sw(e)] = y;
+        Y[sw(ky)] = y;
+    };
+#pragma unroll
+    for (int i = 0; i < kPre; ++i) {
+        const int ky = tid + i * T;
+        if (ky < H) row(ky, pah[i], pd[i], ps[i]);
+    }
+    for (int ky = tid + kPre * T; ky < H; ky += T) {
+        float2 ah;
+        float d[16], sk[4];
+        load_y(ky, ah, d, sk);
+        row(ky, ah, d, sk);
     }
     // reduce the partial sums across the workgroup (the LDS after Y); the
     // barrier inside also orders the Y stores before the inverse transform
@@ -330,12 +408,16 @@ hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float
 }
 
 hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *U, const float *A, const float *g,
-                          float *gpb, float *FG, double *accRows, hipStream_t s) {
+                          const float *G1, float *gpb, float *FG, double *accRows, hipStream_t s) {
     const size_t lds = adj_lds_rows(a, T);
     return by_threads(T, [&](auto t) {
         constexpr int TT = decltype(t)::value;
-        hipLaunchKernelGGL(k_adj_mid<TT>, dim3(a.H / 2), dim3(TT), lds, s, a, U, A, g, gpb, FG,
-                           accRows);
+        if (G1)
+            hipLaunchKernelGGL((k_adj_mid<TT, true>), dim3(a.H / 2), dim3(TT), lds, s, a, U, A, g, G1,
+                               gpb, FG, accRows);
+        else
+            hipLaunchKernelGGL((k_adj_mid<TT, false>), dim3(a.H / 2), dim3(TT), lds, s, a, U, A, g, G1,
+                               gpb, FG, accRows);
     });
 }
 

@@ -92,8 +92,10 @@ size_t mfma_lds_bytes(int h, int W, bool split);
 // adjoint (k_adjoint.hip)
 hipError_t launch_adj_edge(const AdjArgs &a, int T, const float *G1, const float *gpb, float *gOut,
                            hipStream_t s);
+// G1 non-NULL: the incoming gradient is gpb + row inverse of G1 (the later
+// step's edge fused in); NULL: it is read from g
 hipError_t launch_adj_mid(const AdjArgs &a, int T, const float *U, const float *A, const float *g,
-                          float *gpb, float *FG, double *accRows, hipStream_t s);
+                          const float *G1, float *gpb, float *FG, double *accRows, hipStream_t s);
 hipError_t launch_adj_cols(const AdjArgs &a, int T, const float *Fin, const float *Ahat, float *G,
                            double *accCols, hipStream_t s);
 hipError_t launch_adj_final(const double *accRows, int nRows, int nvRows, const double *accCols,

@@ -9,7 +9,7 @@
 // steps are recomputed bit-exactly from its checkpoint with the normal step
 // kernels, which (through the save pointers of a copy of the world record)
 // also leave each step's input state, potentials and 2D spectrum; the steps
-// are then reversed from the segment's end to its start with three launches
+// are then reversed from the segment's end to its start with two launches
 // each (k_adjoint.hip).  Both sequences are captured once per (T, c) into
 // hipGraphs and replayed; every kernel reads the growth parameters from the
 // world's device record, so parameter updates between calls keep the graphs
@@ -302,7 +302,10 @@ static hipError_t recompute_segment(lenia_world *w, int sg, hipStream_t s) {
     return e;
 }
 
-// reverse the steps of segment sg from its saves, last step first
+// reverse the steps of segment sg from its saves, last step first.  Step t's
+// row launch also finishes step t+1 (its edge: gp + row inverse of G1), so a
+// reverse step is two launches; the edge of step 0 runs on its own after the
+// last segment.
 static hipError_t reverse_segment(lenia_world *w, int sg, hipStream_t s) {
     AdjWorkspace *ws = w->adj;
     const AdjArgs &a = ws->args;
@@ -312,13 +315,14 @@ static hipError_t reverse_segment(lenia_world *w, int sg, hipStream_t s) {
     hipError_t e = hipSuccess;
     for (int j = (int)m - 1; j >= 0 && e == hipSuccess; --j) {
         const size_t slot = base + j;
+        const bool last = t0 + j == T - 1;  // its incoming gradient is dL/dA_T in g
         e = launch_adj_mid(a, ws->rowT, ws->saveU + slot * a.K * n, ws->saveA + slot * n, ws->g,
-                           ws->gpb, ws->FG, ws->accRows, s);
+                           last ? nullptr : ws->G1, ws->gpb, ws->FG, ws->accRows, s);
         if (e == hipSuccess)
             e = launch_adj_cols(a, ws->colT, ws->FG, ws->saveAhat + slot * ws->spec, ws->G1,
                                 ws->accCols, s);
-        if (e == hipSuccess) e = launch_adj_edge(a, ws->rowT, ws->G1, ws->gpb, ws->g, s);
     }
+    if (e == hipSuccess && sg == 0) e = launch_adj_edge(a, ws->rowT, ws->G1, ws->gpb, ws->g, s);
     return e;
 }
 
