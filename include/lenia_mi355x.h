@@ -274,8 +274,10 @@ int lenia_batch_destroy(lenia_batch *b);
  * hipGraph once and replays it (nSteps must then be a multiple of graphSteps) */
 int lenia_batch_step(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
 /* the batch's launch shapes (one workgroup size and side-by-side choice per
- * pass for all worlds) picked by timing every allowed choice over nSteps steps;
- * states and clocks restored, results unchanged bit for bit */
+ * pass for all worlds) picked by timing every allowed choice over nSteps steps,
+ * then the transform executors, whose finalists are timed as graph-replayed
+ * steps; states and clocks restored, results unchanged bit for bit.
+ * *msPerStep: the chosen configuration's graph-replayed time per step */
 int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep);
 /* FFT-form launch grids (no counterpart in the reference).  Withdrawn: the
  * resident tile-queue grids ran 3x slower than one workgroup per tile on the

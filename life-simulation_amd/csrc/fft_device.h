@@ -231,13 +231,15 @@ constexpr bool stage_fold() {
 template <int S>
 __device__ __forceinline__ constexpr int stage_off(int r) { return r * S + (r * S) / 32; }
 
-// One Stockham stage over nt transforms in LDS (padded placement).  Every thread
-// first loads all its butterflies, the group synchronises, then each thread
-// twiddles, transforms and stores, so the stage runs in place in one LDS
-// buffer.  `tws` is this stage's twiddle table, [k][r-1] for k < Ns,
-// 0 < r < R.  Precondition (host): ceil(nt*N/R / T) * R <= MAXV.
+// One Stockham stage over nt transforms in LDS (padded placement), from buf
+// to dst.  Every thread loads its butterflies, twiddles, transforms and
+// stores.  In place (dst == buf) the group synchronises between the loads and
+// the stores; between two regions (dst != buf) it does not, since no thread
+// writes what another still reads: one barrier per stage instead of two.
+// `tws` is this stage's twiddle table, [k][r-1] for k < Ns, 0 < r < R.
+// Precondition (host): ceil(nt*N/R / T) * R <= MAXV.
 template <int R, bool INV, int MAXV, int T>
-__device__ __forceinline__ void fft_stage(float2 *buf, const FftStage &st, int N, int nt,
+__device__ __forceinline__ void fft_stage(float2 *buf, float2 *dst, const FftStage &st, int N, int nt,
                                           const float2 *tws, int tid) {
     constexpr int MB = MAXV / R;
     if constexpr (MB > 0) {
@@ -262,7 +264,7 @@ __device__ __forceinline__ void fft_stage(float2 *buf, const FftStage &st, int N
                 for (int r = 0; r < R - 1; ++r) w[i][r] = t[r];
             }
         }
-        __syncthreads();
+        if (dst == buf) __syncthreads();
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
             int g = tid + i * T;
@@ -277,7 +279,7 @@ __device__ __forceinline__ void fft_stage(float2 *buf, const FftStage &st, int N
                 Dft<R, INV>::run(v[i]);
                 const int base = b * N + (j - k) * R + k;
 #pragma unroll
-                for (int r = 0; r < R; ++r) buf[sw(base + r * Ns)] = v[i][r];
+                for (int r = 0; r < R; ++r) dst[sw(base + r * Ns)] = v[i][r];
             }
         }
         __syncthreads();
@@ -286,23 +288,30 @@ __device__ __forceinline__ void fft_stage(float2 *buf, const FftStage &st, int N
 
 // Full transform; the caller synchronises after filling `buf` and the LDS
 // stage-twiddle tables `tw` (plan.st[s].twOff indexes them).  Unscaled both
-// ways.
+// ways.  The result is left in `buf`.  `alt` (NULL: none) is a second region
+// of the same span: the stages then alternate between the two, ending in
+// `buf` (with an odd stage count the first stage runs in place).  Placement
+// never changes arithmetic, so both forms give the same bits.
 template <bool INV, int MAXV, int T>
 __device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
-                                        int tid) {
+                                        int tid, float2 *alt = nullptr) {
     const int N = plan.n;
+    if (!alt) alt = buf;
+    float2 *src = buf;
     for (int s = 0; s < plan.nStages; ++s) {
         const FftStage &st = plan.st[s];
         const float2 *tws = tw + st.twOff;
+        float2 *dst = ((plan.nStages - 1 - s) & 1) ? alt : buf;
         switch (st.radix) {
-            case 2: fft_stage<2, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            case 3: fft_stage<3, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            case 4: fft_stage<4, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            case 5: fft_stage<5, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            case 8: fft_stage<8, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            case 16: fft_stage<16, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
-            default: fft_stage<32, INV, MAXV, T>(buf, st, N, nt, tws, tid); break;
+            case 2: fft_stage<2, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            case 3: fft_stage<3, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            case 4: fft_stage<4, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            case 5: fft_stage<5, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            case 8: fft_stage<8, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            case 16: fft_stage<16, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
+            default: fft_stage<32, INV, MAXV, T>(src, dst, st, N, nt, tws, tid); break;
         }
+        src = dst;
     }
 }
 
@@ -311,7 +320,8 @@ __device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan &plan, int nt
 // and 5), the stage tables need no loads and there is no radix switch.  The
 // butterflies, twiddles and stores are those of fft_stage, so the bits are too.
 template <int R, bool INV, int MAXV, int T, int N, int NS, int TWOFF>
-__device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *tw, int tid) {
+__device__ __forceinline__ void fft_stage_c(float2 *buf, float2 *dstBuf, int nt, const float2 *tw,
+                                            int tid) {
     constexpr int NR = N / R;
     constexpr int MB = MAXV / R;
     static_assert(N % (NS * R) == 0, "radix sequence does not divide the length");
@@ -340,7 +350,7 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
                 for (int r = 0; r < R - 1; ++r) w[i][r] = t[r];
             }
         }
-        __syncthreads();
+        if (dstBuf == buf) __syncthreads();
 #pragma unroll
         for (int i = 0; i < MB; ++i) {
             const int g = tid + i * T;
@@ -353,12 +363,12 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
                 Dft<R, INV>::run(v[i]);
                 const int base = b * N + (j - k) * R + k;
                 if constexpr (kFoldSt) {
-                    float2 *dst = buf + sw(base);
+                    float2 *dst = dstBuf + sw(base);
 #pragma unroll
                     for (int r = 0; r < R; ++r) dst[stage_off<NS>(r)] = v[i][r];
                 } else {
 #pragma unroll
-                    for (int r = 0; r < R; ++r) buf[sw(base + r * NS)] = v[i][r];
+                    for (int r = 0; r < R; ++r) dstBuf[sw(base + r * NS)] = v[i][r];
                 }
             }
         }
@@ -366,11 +376,19 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, int nt, const float2 *t
     }
 }
 
+// stages from src; each stage's destination alternates so the last one ends
+// in buf (alt == buf: all in place)
 template <bool INV, int MAXV, int T, int N, int NS, int TWOFF, int R, int... REST>
-__device__ __forceinline__ void fft_fixed(float2 *buf, int nt, const float2 *tw, int tid) {
-    fft_stage_c<R, INV, MAXV, T, N, NS, TWOFF>(buf, nt, tw, tid);
+__device__ __forceinline__ void fft_fixed_from(float2 *buf, float2 *alt, float2 *src, int nt,
+                                               const float2 *tw, int tid) {
+    float2 *dst = (sizeof...(REST) & 1) ? alt : buf;
+    fft_stage_c<R, INV, MAXV, T, N, NS, TWOFF>(src, dst, nt, tw, tid);
     if constexpr (sizeof...(REST) > 0)
-        fft_fixed<INV, MAXV, T, N, NS * R, TWOFF + NS * (R - 1), REST...>(buf, nt, tw, tid);
+        fft_fixed_from<INV, MAXV, T, N, NS * R, TWOFF + NS * (R - 1), REST...>(buf, alt, dst, nt, tw, tid);
+}
+template <bool INV, int MAXV, int T, int N, int NS, int TWOFF, int R, int... REST>
+__device__ __forceinline__ void fft_fixed(float2 *buf, float2 *alt, int nt, const float2 *tw, int tid) {
+    fft_fixed_from<INV, MAXV, T, N, NS, TWOFF, R, REST...>(buf, alt ? alt : buf, buf, nt, tw, tid);
 }
 
 // fft_lds through the compile-time kernel of the plan when it has one
@@ -378,51 +396,52 @@ __device__ __forceinline__ void fft_fixed(float2 *buf, int nt, const float2 *tw,
 // out of the default plans' code path
 template <bool INV, int MAXV, int T>
 __device__ __forceinline__ void fft_run_alt(float2 *buf, uint32_t code, int nt,
-                                                      const float2 *tw, int tid) {
+                                                      const float2 *tw, int tid, float2 *alt) {
     switch (code) {
-        case kPlan64b: fft_fixed<INV, MAXV, T, 64, 1, 0, 16, 4>(buf, nt, tw, tid); break;
-        case kPlan64c: fft_fixed<INV, MAXV, T, 64, 1, 0, 4, 16>(buf, nt, tw, tid); break;
-        case kPlan128b: fft_fixed<INV, MAXV, T, 128, 1, 0, 8, 16>(buf, nt, tw, tid); break;
-        case kPlan128c: fft_fixed<INV, MAXV, T, 128, 1, 0, 4, 4, 8>(buf, nt, tw, tid); break;
-        case kPlan256b: fft_fixed<INV, MAXV, T, 256, 1, 0, 8, 8, 4>(buf, nt, tw, tid); break;
-        case kPlan256c: fft_fixed<INV, MAXV, T, 256, 1, 0, 4, 8, 8>(buf, nt, tw, tid); break;
-        case kPlan512b: fft_fixed<INV, MAXV, T, 512, 1, 0, 16, 8, 4>(buf, nt, tw, tid); break;
-        case kPlan512c: fft_fixed<INV, MAXV, T, 512, 1, 0, 4, 8, 16>(buf, nt, tw, tid); break;
-        case kPlan1024b: fft_fixed<INV, MAXV, T, 1024, 1, 0, 8, 8, 16>(buf, nt, tw, tid); break;
-        case kPlan1024c: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 16, 4>(buf, nt, tw, tid); break;
-        case kPlan1024d: fft_fixed<INV, MAXV, T, 1024, 1, 0, 4, 16, 16>(buf, nt, tw, tid); break;
-        case kPlan2048b: fft_fixed<INV, MAXV, T, 2048, 1, 0, 8, 16, 16>(buf, nt, tw, tid); break;
-        case kPlan4096b: fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, nt, tw, tid); break;
-        case kPlan96b: fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, nt, tw, tid); break;
-        case kPlan192b: fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, nt, tw, tid); break;
-        default: fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, nt, tw, tid); break;
+        case kPlan64b: fft_fixed<INV, MAXV, T, 64, 1, 0, 16, 4>(buf, alt, nt, tw, tid); break;
+        case kPlan64c: fft_fixed<INV, MAXV, T, 64, 1, 0, 4, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan128b: fft_fixed<INV, MAXV, T, 128, 1, 0, 8, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan128c: fft_fixed<INV, MAXV, T, 128, 1, 0, 4, 4, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan256b: fft_fixed<INV, MAXV, T, 256, 1, 0, 8, 8, 4>(buf, alt, nt, tw, tid); break;
+        case kPlan256c: fft_fixed<INV, MAXV, T, 256, 1, 0, 4, 8, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan512b: fft_fixed<INV, MAXV, T, 512, 1, 0, 16, 8, 4>(buf, alt, nt, tw, tid); break;
+        case kPlan512c: fft_fixed<INV, MAXV, T, 512, 1, 0, 4, 8, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan1024b: fft_fixed<INV, MAXV, T, 1024, 1, 0, 8, 8, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan1024c: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 16, 4>(buf, alt, nt, tw, tid); break;
+        case kPlan1024d: fft_fixed<INV, MAXV, T, 1024, 1, 0, 4, 16, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan2048b: fft_fixed<INV, MAXV, T, 2048, 1, 0, 8, 16, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan4096b: fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan96b: fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, alt, nt, tw, tid); break;
+        case kPlan192b: fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, alt, nt, tw, tid); break;
+        default: fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, alt, nt, tw, tid); break;
     }
 }
 
 // fft_lds through the compile-time kernel of the plan when it has one
+// alt: an optional second LDS region of the same span (see fft_lds)
 template <bool INV, int MAXV, int T>
 __device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
-                                        int tid) {
+                                        int tid, float2 *alt = nullptr) {
     if (__builtin_expect(plan.code >= kPlan64b, 0)) {
-        fft_run_alt<INV, MAXV, T>(buf, plan.code, nt, tw, tid);
+        fft_run_alt<INV, MAXV, T>(buf, plan.code, nt, tw, tid, alt);
         return;
     }
     switch (plan.code) {
-        case kPlan64: fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, nt, tw, tid); break;
-        case kPlan128: fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, nt, tw, tid); break;
-        case kPlan256: fft_fixed<INV, MAXV, T, 256, 1, 0, 16, 16>(buf, nt, tw, tid); break;
-        case kPlan512: fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, nt, tw, tid); break;
-        case kPlan1024: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, nt, tw, tid); break;
-        case kPlan2048: fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, nt, tw, tid); break;
-        case kPlan96: fft_fixed<INV, MAXV, T, 96, 1, 0, 8, 4, 3>(buf, nt, tw, tid); break;
-        case kPlan192: fft_fixed<INV, MAXV, T, 192, 1, 0, 8, 8, 3>(buf, nt, tw, tid); break;
-        case kPlan384: fft_fixed<INV, MAXV, T, 384, 1, 0, 16, 8, 3>(buf, nt, tw, tid); break;
-        case kPlan720: fft_fixed<INV, MAXV, T, 720, 1, 0, 16, 3, 3, 5>(buf, nt, tw, tid); break;
-        case kPlan1080: fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, nt, tw, tid); break;
-        case kPlan1280: fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, nt, tw, tid); break;
-        case kPlan1920: fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, nt, tw, tid); break;
-        case kPlan4096: fft_fixed<INV, MAXV, T, 4096, 1, 0, 16, 16, 16>(buf, nt, tw, tid); break;
-        default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid); break;
+        case kPlan64: fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan128: fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan256: fft_fixed<INV, MAXV, T, 256, 1, 0, 16, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan512: fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan1024: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan2048: fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, alt, nt, tw, tid); break;
+        case kPlan96: fft_fixed<INV, MAXV, T, 96, 1, 0, 8, 4, 3>(buf, alt, nt, tw, tid); break;
+        case kPlan192: fft_fixed<INV, MAXV, T, 192, 1, 0, 8, 8, 3>(buf, alt, nt, tw, tid); break;
+        case kPlan384: fft_fixed<INV, MAXV, T, 384, 1, 0, 16, 8, 3>(buf, alt, nt, tw, tid); break;
+        case kPlan720: fft_fixed<INV, MAXV, T, 720, 1, 0, 16, 3, 3, 5>(buf, alt, nt, tw, tid); break;
+        case kPlan1080: fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, alt, nt, tw, tid); break;
+        case kPlan1280: fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, alt, nt, tw, tid); break;
+        case kPlan1920: fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, alt, nt, tw, tid); break;
+        case kPlan4096: fft_fixed<INV, MAXV, T, 4096, 1, 0, 16, 16, 16>(buf, alt, nt, tw, tid); break;
+        default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid, alt); break;
     }
 }
 

@@ -133,11 +133,11 @@ __device__ __forceinline__ void gather_store(float2 *Z, const float4 (&gv)[NG], 
 // forward FFT of the tile's (new) row pairs, split into the two real rows'
 // half-spectra Fa = (Z[k] + conj Z[W-k]) / 2, Fb = (Z[k] - conj Z[W-k]) / 2i
 template <int T>
-__device__ __forceinline__ void forward_rows(float2 *Z, const FftWorldDev &wd, const float2 *tw,
-                                             int rp0, int nt, int tid) {
+__device__ __forceinline__ void forward_rows(float2 *Z, float2 *Zt, const FftWorldDev &wd,
+                                             const float2 *tw, int rp0, int nt, int tid) {
     const int W = wd.W, H2 = wd.H / 2, P = wd.P;
     const int nm = P * wd.Wh;
-    fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid);
+    fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid, Zt);
     LENIA_GLOBAL float4 *F4 = gp(reinterpret_cast<float4 *>(wd.F)) + rp0;
     for (int m = tid; m < nm; m += T) {
         int kx = (int)fast_div((uint32_t)m, wd.mP), p = m - kx * P;
@@ -171,6 +171,8 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     const uint32_t mP = wd.mP;
     float2 *tw = smem;
     float2 *Z = smem + W;
+    // the twin transform region, when the launch has one (rows_twin)
+    float2 *Zt = rows_twin(W, P, wd.K, KPAR, T) ? Z + rows_region(W, P, wd.K, KPAR) : nullptr;
     LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
     const int nq = nt * W;
     const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
@@ -186,7 +188,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
             Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
         }
         __syncthreads();
-        forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+        forward_rows<T>(Z, Zt, wd, tw, rp0, nt, tid);
         __syncthreads();
         return;
     }
@@ -220,7 +222,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         if (newWorld) twc.store(tw, twg, W, tid);
         gather_store<kG>(Z, gv, zi, kxv, W);
         __syncthreads();
-        fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid);
+        fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid, Zt);
         if (MODE == 2) {
             for (int q = tid; q < nq; q += T) {
                 int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
@@ -354,7 +356,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         }
     }
     __syncthreads();
-    forward_rows<T>(Z, wd, tw, rp0, nt, tid);
+    forward_rows<T>(Z, Zt, wd, tw, rp0, nt, tid);
     __syncthreads();
 }
 
@@ -422,7 +424,8 @@ __device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds
             }
         }
         __syncthreads();
-        fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid);
+        // Y is free until the scaled copies: the forward transform's twin region
+        fft_run<false, kMaxV, T>(X, wd.planH, nt, tw, tid, Y);
         if (wd.saveAhat) {
             LENIA_GLOBAL float2 *sv = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + (size_t)c0 * H;
             for (int e = tid; e < ne; e += T) gst2(sv + e, X[sw(e)]);

@@ -105,11 +105,11 @@ int compiled_plans(int n, uint32_t *packed, int maxOut);
 // the plan cache's entry for (axis 0 rows / 1 columns, length n), 0 if none
 uint32_t plan_cache_lookup(uint32_t axis, uint32_t n);
 bool plan_fits(const FftPlan &p, int nt, int T, int maxv);
-size_t lds_rows(int W, int P, int K, bool kpar);
+// LDS of a row-pass launch of T threads (with the twin transform region of
+// rows_twin when it applies); of a column-pass launch.  kFftLdsMax
+// (lenia_types.h) is the limit of FFT shapes.
+size_t lds_rows(int W, int P, int K, bool kpar, int T);
 size_t lds_cols(int H, int C, int K, bool kpar);
-// LDS a gfx950 workgroup can address (160 KB) less what the step kernels
-// declare statically (the tile-queue slot): the limit of FFT shapes
-constexpr size_t kFftLdsMax = 163840 - 256;
 bool shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpar);
 bool pick_shape(const FftPlan &p, bool rows, int count, int K, uint32_t wantNt, uint32_t wantT,
                 int &ntOut, int &TOut, bool &kparOut);

@@ -195,7 +195,8 @@ static int batch_setup_fft(lenia_batch *b) {
             rlds[c] = std::max(rlds[c], rs ? wave_split_lds_bytes(f.W, Lr, true) : wave_lds_bytes(f.W, Lr, b->rowWaveNW));
         } else {
             for (int t = 0; t < f.rowTiles; ++t) rt.push_back(make_int2(i, t));
-            b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar), lds_rows(f.W, f.P, 1, true)});
+            b->ldsRows = std::max({b->ldsRows, lds_rows(f.W, f.P, K, b->rowKpar, b->rowT),
+                                   lds_rows(f.W, f.P, 1, true, b->rowT)});
             b->rowCells = std::max(b->rowCells, (b->rowKpar ? K : 1) * f.P * f.W);
         }
         if (Lc) {
@@ -416,6 +417,8 @@ static int batch_prepare(lenia_batch *b) {
 // workgroup size and side-by-side choice are timed over nSteps steps with
 // per-kernel timestamps; the worlds' states and clocks are restored.  Plans
 // do not change, so results stay bit-identical to stepping each world alone.
+static int batch_run_steps(lenia_batch *b, uint32_t nSteps, uint32_t graphSteps);
+
 extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msPerStep) {
     if (!b) return fail(LENIA_ERR_INVALID_ARG, "lenia_batch_autotune: NULL batch");
     if (!b->isFft) {
@@ -513,6 +516,41 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
             bestRowNW = nw;
         }
     }
+    // The per-pass times above come from launches timed one by one; the
+    // batch runs graph-replayed, where the side-by-side launches of the
+    // executors overlap differently.  The finalists are timed that way and
+    // the fastest whole step is kept.
+    double graphStepMs = 0.0;
+    if (!rc && bestRowT && bestColT) {
+        std::vector<std::pair<int, int>> cand = {{bestRowNW, bestColNW}, {0, 0}};
+        for (int ni = 0; ni < 4; ++ni) cand.push_back({bestRowNW, kNw[ni]});
+        double bestStep = 1e30;
+        std::pair<int, int> pick = cand[0];
+        std::vector<uint64_t> hostSteps;
+        for (lenia_world *w : b->worlds) hostSteps.push_back(w->hostSteps);
+        constexpr uint32_t kG = 8, kRuns = 4;
+        for (size_t i = 0; i < cand.size(); ++i) {
+            if (std::find(cand.begin(), cand.begin() + i, cand[i]) != cand.begin() + i) continue;
+            b->rowWaveNW = cand[i].first;
+            b->colWaveNW = cand[i].second;
+            if (batch_setup(b) != LENIA_OK) continue;
+            if (batch_run_steps(b, kG, kG) != LENIA_OK) continue;  // capture, warm-up
+            if (batch_run_steps(b, kRuns * kG, kG) != LENIA_OK) continue;
+            float ms = 0.f;
+            if (hipEventSynchronize(ctx->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess)
+                continue;
+            const double per = ms / (kRuns * kG);
+            if (per < bestStep) {
+                bestStep = per;
+                pick = cand[i];
+            }
+        }
+        for (size_t i = 0; i < b->worlds.size(); ++i) b->worlds[i]->hostSteps = hostSteps[i];
+        bestRowNW = pick.first;
+        bestColNW = pick.second;
+        if (bestStep < 1e30) graphStepMs = bestStep;
+    }
     b->rowWaveNW = bestRowNW;
     b->colWaveNW = bestColNW;
     const int rc2 = batch_setup(b);
@@ -536,7 +574,7 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     if (!bestRowT || !bestColT) return fail(LENIA_ERR_UNSUPPORTED, "batch autotune: no valid shape");
     if (rc2) return rc2;
     if (e != hipSuccess) return hip_fail(e, "batch autotune restore");
-    if (msPerStep) *msPerStep = best[0] + best[1];
+    if (msPerStep) *msPerStep = graphStepMs > 0.0 ? graphStepMs : best[0] + best[1];
     return LENIA_OK;
 }
 

@@ -560,8 +560,9 @@ static int build_kernels_fft(lenia_world *w) {
 
 // Transform regions span lds_span(n) entries (one unused entry after every
 // 32, the LDS placement of the kernels).
-size_t lenia::lds_rows(int W, int P, int K, bool kpar) {
-    return (size_t)(W + lds_span((kpar ? K : 1) * P * W)) * sizeof(float2);
+size_t lenia::lds_rows(int W, int P, int K, bool kpar, int T) {
+    const int twin = rows_twin(W, P, K, kpar, T) ? 2 : 1;
+    return (size_t)(W + twin * rows_region(W, P, K, kpar)) * sizeof(float2);
 }
 size_t lenia::lds_cols(int H, int C, int K, bool kpar) {
     return (size_t)(H + lds_span(C * H) + (kpar ? lds_span(K * C * H) : 0)) * sizeof(float2);
@@ -573,7 +574,7 @@ bool lenia::shape_ok(const FftPlan &p, bool rows, int nt, int K, int T, bool kpa
     // 1024-thread workgroups: the row pass with side-by-side transforms only
     if (T > 512 && (!rows || !kpar)) return false;
     const int mv = fft_max_values();
-    const size_t lds = rows ? lds_rows(p.n, nt, K, kpar) : lds_cols(p.n, nt, K, kpar);
+    const size_t lds = rows ? lds_rows(p.n, nt, K, kpar, T) : lds_cols(p.n, nt, K, kpar);
     if (lds > kFftLdsMax) return false;
     if (rows && (long)(kpar ? nt * K : nt) * p.n > (long)T * fft_row_values(T)) return false;
     // side-by-side gather: each thread loads V/2 + 1 spectrum slots, and the
@@ -757,7 +758,7 @@ FftWorldDev lenia::fft_record(lenia_world *w) {
         w->ldsRows = w->rowSplit ? std::max(wave_split_lds_bytes(f.W, Lw, true), wave_lds_bytes(f.W, Lw, 1))
                                  : wave_lds_bytes(f.W, Lw, w->rowNW);
     else
-        w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar), lds_rows(f.W, f.P, 1, true));
+        w->ldsRows = std::max(lds_rows(f.W, f.P, f.K, w->rowKpar, w->rowT), lds_rows(f.W, f.P, 1, true, w->rowT));
     if (w->colWave)
         w->ldsCols = w->colSplit ? wave_split_lds_bytes(f.H, Lh, false) : wave_lds_bytes(f.H, Lh, w->colNW);
     else

@@ -14,6 +14,19 @@ constexpr int kMaxStages = 8;  // FFT plan: up to 8 radix stages
 // placement i -> i + i/32 leaves one unused entry after every 32
 constexpr int lds_span(int n) { return n + (n + 31) / 32; }
 
+// LDS bytes one workgroup may use (gfx950: 160 KiB, less a margin)
+constexpr size_t kFftLdsMax = 163840 - 256;
+
+// The row pass at 1024 threads runs its transforms between two LDS regions
+// (one barrier per stage instead of two) when the second region fits: the
+// twiddles [W], the transform region, then its twin.  Host and kernel both
+// decide by this rule, so the launch's LDS always covers what the kernel
+// uses.
+constexpr int rows_region(int W, int P, int K, bool kpar) { return lds_span((kpar ? K : 1) * P * W); }
+constexpr bool rows_twin(int W, int P, int K, bool kpar, int T) {
+    return T >= 1024 && (size_t)(W + 2 * rows_region(W, P, K, kpar)) * 8u <= kFftLdsMax;
+}
+
 // cosine-palette colour map (ColorMapParams, LifeCore/Render/ColorMapPass.h:16-25;
 // same layout as lenia_colormap in the C ABI)
 struct ColorMapParams {
