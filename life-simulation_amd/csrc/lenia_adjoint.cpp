@@ -32,10 +32,11 @@ struct AdjWorkspace {
     size_t spec = 0;  // floats of one complex half spectrum
     int K = 0, T = 0, c = 0, nShells = 0;
     float *ckpt = nullptr;  // [ceil(T/c)][n] states at t = 0, c, 2c, ...
-    // two sets of c slots: segment sg recomputes into set sg % 2 while the
-    // reverse steps of segment sg + 1 read the other set
-    float *saveA = nullptr, *saveU = nullptr, *saveAhat = nullptr;  // [2c] slots
-    FftWorldDev *recs = nullptr;  // [2c] world records with save pointers
+    // three sets of c slots: segment sg < last recomputes into set sg % 2
+    // while the reverse steps of segment sg + 1 read the other set; set 2
+    // holds the last segment, saved by the forward sweep itself
+    float *saveA = nullptr, *saveU = nullptr, *saveAhat = nullptr;  // [3c] slots
+    FftWorldDev *recs = nullptr;  // [3c] world records with save pointers
     hipStream_t side = nullptr;   // the recompute branch of the captured graph
     hipEvent_t fork = nullptr, join = nullptr;
     float *FG = nullptr, *G1 = nullptr, *gpb = nullptr, *g = nullptr, *D = nullptr;
@@ -194,10 +195,10 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
         a.twH = w->twH;
         a.S = w->S;
         int rc = dev_alloc(ctx, (void **)&ws->ckpt, sizeof(float) * n * nck);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveA, sizeof(float) * n * 2 * c);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveU, sizeof(float) * n * K * 2 * c);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveAhat, sizeof(float) * spec * 2 * c);
-        if (!rc) rc = dev_alloc(ctx, (void **)&ws->recs, sizeof(FftWorldDev) * 2 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveA, sizeof(float) * n * 3 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveU, sizeof(float) * n * K * 3 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->saveAhat, sizeof(float) * spec * 3 * c);
+        if (!rc) rc = dev_alloc(ctx, (void **)&ws->recs, sizeof(FftWorldDev) * 3 * c);
         if (!rc) {
             hipError_t e = hipStreamCreateWithFlags(&ws->side, hipStreamNonBlocking);
             if (e == hipSuccess) e = hipEventCreateWithFlags(&ws->fork, hipEventDisableTiming);
@@ -237,6 +238,7 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
         };
         ws->rowT = pick(w->planW, K, true);
         ws->colT = pick(w->planH, K, false);
+
         if (!ws->rowT || !ws->colT) {
             world_free_adjoint(w);
             return fail(LENIA_ERR_UNSUPPORTED, "rollout gradients: field %dx%d with %d kernels does "
@@ -268,14 +270,25 @@ static hipError_t load_state(lenia_world *w, const float *src, hipStream_t s) {
     return e;
 }
 
-// forward sweep from ckpt[0]: checkpoints at t = c, 2c, ...; the world ends at
-// A_T with its clock T steps on
+// records with save pointers for the 3c slots (refreshed from the world
+// record, so parameter updates are seen)
+static hipError_t enqueue_records(lenia_world *w, hipStream_t s) {
+    AdjWorkspace *ws = w->adj;
+    return launch_adj_records((const FftWorldDev *)w->devDesc, ws->recs, 3 * ws->c, ws->saveA, ws->saveU,
+                              ws->saveAhat, ws->n, ws->spec, s);
+}
+
+// forward sweep from ckpt[0]: checkpoints at t = c, 2c, ...; the steps of the
+// last segment also leave their saves (slot set 2), which the reverse pass
+// reads first instead of recomputing them; the world ends at A_T with its
+// clock T steps on
 static hipError_t enqueue_forward(lenia_world *w) {
     AdjWorkspace *ws = w->adj;
     hipStream_t s = w->ctx->stream;
     const size_t n = ws->n;
     const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
-    hipError_t e = load_state(w, ws->ckpt, s);
+    hipError_t e = enqueue_records(w, s);
+    if (e == hipSuccess) e = load_state(w, ws->ckpt, s);
     uint32_t t = 0;
     for (uint32_t k = 1; k < nck && e == hipSuccess; ++k) {
         for (uint32_t i = 0; i < c && e == hipSuccess; ++i) e = enqueue_step(w, nullptr, t++);
@@ -283,7 +296,11 @@ static hipError_t enqueue_forward(lenia_world *w) {
             e = hipMemcpyAsync(ws->ckpt + n * k, w->current_state(), sizeof(float) * n,
                                hipMemcpyDeviceToDevice, s);
     }
-    while (t < T && e == hipSuccess) e = enqueue_step(w, nullptr, t++);
+    const FftWorldDev *last = ws->recs + (size_t)2 * c;
+    for (uint32_t j = 0; t < T && e == hipSuccess; ++j, ++t) {
+        e = world_launch_cols(w, last + j, s);
+        if (e == hipSuccess) e = world_launch_rows(w, 1, last + j, t, 0, nullptr, false, s);
+    }
     if (e == hipSuccess) e = world_advance_clock(w, T);
     return e;
 }
@@ -311,13 +328,14 @@ static hipError_t reverse_segment(lenia_world *w, int sg, hipStream_t s) {
     const AdjArgs &a = ws->args;
     const size_t n = ws->n;
     const uint32_t T = ws->T, c = ws->c, t0 = sg * c, m = std::min(T, t0 + c) - t0;
-    const size_t base = (size_t)(sg % 2) * c;
+    const bool last = t0 + m == T;  // saved by the forward sweep
+    const size_t base = (size_t)(last ? 2 : sg % 2) * c;
     hipError_t e = hipSuccess;
     for (int j = (int)m - 1; j >= 0 && e == hipSuccess; --j) {
         const size_t slot = base + j;
-        const bool last = t0 + j == T - 1;  // its incoming gradient is dL/dA_T in g
+        const bool first = t0 + j == T - 1;  // its incoming gradient is dL/dA_T in g
         e = launch_adj_mid(a, ws->rowT, ws->saveU + slot * a.K * n, ws->saveA + slot * n, ws->g,
-                           last ? nullptr : ws->G1, ws->gpb, ws->FG, ws->accRows, s);
+                           first ? nullptr : ws->G1, ws->gpb, ws->FG, ws->accRows, s);
         if (e == hipSuccess)
             e = launch_adj_cols(a, ws->colT, ws->FG, ws->saveAhat + slot * ws->spec, ws->G1,
                                 ws->accCols, s);
@@ -326,22 +344,20 @@ static hipError_t reverse_segment(lenia_world *w, int sg, hipStream_t s) {
     return e;
 }
 
-// the reverse pass from the checkpoints and g = dL/dA_T: leaves the
-// parameter sums in acc and dL/dA0 in g, the world at A0.  The recompute of
-// segment sg - 1 (side stream) runs beside the reverse steps of segment sg.
+// the reverse pass from the checkpoints, the last segment's saves (left by
+// the forward sweep) and g = dL/dA_T: leaves the parameter sums in acc and
+// dL/dA0 in g, the world at A0.  The recompute of segment sg - 1 (side
+// stream) runs beside the reverse steps of segment sg.
 static hipError_t enqueue_backward(lenia_world *w) {
     AdjWorkspace *ws = w->adj;
     const AdjArgs &a = ws->args;
     hipStream_t s = w->ctx->stream;
     const uint32_t T = ws->T, c = ws->c, nck = (T + c - 1) / c;
     const int H2 = a.H / 2, Wh = a.W / 2 + 1;
-    // records with save pointers (refreshed from the world record every call,
-    // so parameter updates are seen) and zeroed FP64 slots (fill kernels: graph-safe)
-    hipError_t e = launch_adj_records((const FftWorldDev *)w->devDesc, ws->recs, (int)(2 * c), ws->saveA,
-                                      ws->saveU, ws->saveAhat, ws->n, ws->spec, s);
+    // records with save pointers and zeroed FP64 slots (fill kernels: graph-safe)
+    hipError_t e = enqueue_records(w, s);
     if (e == hipSuccess) e = launch_fill(reinterpret_cast<float *>(ws->accRows), (size_t)32 * H2, 0.0f, s);
     if (e == hipSuccess) e = launch_fill(reinterpret_cast<float *>(ws->accCols), (size_t)32 * Wh, 0.0f, s);
-    if (e == hipSuccess) e = recompute_segment(w, (int)nck - 1, s);
     for (int sg = (int)nck - 1; sg >= 0 && e == hipSuccess; --sg) {
         if (sg > 0) {
             e = hipEventRecord(ws->fork, s);
