@@ -102,6 +102,13 @@ __device__ __forceinline__ void wave_fft(float2 *v, float2 *xb, const float2 *tw
 template <int N>
 constexpr int wave_group_span() { return N + N / 32; }
 
+// twiddle tables (n - 1 entries) of the plan -> LDS; every wave waits for them
+__device__ __forceinline__ void wave_load_twiddles(float2 *tw, const float *src, int n, int nThreads) {
+    const LENIA_GLOBAL float2 *g = gp(reinterpret_cast<const float2 *>(src));
+    for (int t = (int)threadIdx.x; t < n - 1; t += nThreads) tw[t] = gld2(g + t);
+    __syncthreads();
+}
+
 // --------------------------------------------------------------- row pass
 // Where a group's row pair lives: row pair p (live: p < H/2; idle groups
 // run a valid row pair, pc, and store nothing).
@@ -380,8 +387,8 @@ __device__ __forceinline__ void wave_rows_split(const FftWorldDev &wd, int2 tile
 // --------------------------------------------------------------- column pass
 // Tile: C = NW M columns; wave w, group g runs column tile.y C + w M + g.
 template <int N, int L, int... RS>
-__device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, const float2 *tw,
-                                          float2 *xbw, int wave, int lane) {
+__device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, float2 *tw,
+                                          float2 *xbw, int wave, int lane, int nThreads) {
     constexpr int V = N / L, M = 64 / L, H = N;
     const int g = lane / L, l = lane % L;
     float2 *xb = xbw + g * wave_group_span<N>();
@@ -399,6 +406,8 @@ __device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, cons
     for (int m = 0; m < V; ++m) x[m] = gld2(F + L * m);
 #pragma unroll
     for (int m = 0; m < V; ++m) sv[m] = S[L * m];
+    // the twiddle table, its loads in flight together with the column's
+    wave_load_twiddles(tw, wd.twH, H, nThreads);
     wave_fft<N, L, false, 1, 0, RS...>(x, xb, tw, l);
     if (wd.saveAhat && live) {
         LENIA_GLOBAL float2 *sva = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + off;
@@ -429,7 +438,7 @@ __device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, cons
 // one barrier wave k < K multiplies by kernel k's spectrum, transforms back
 // and stores G_k.
 template <int N, int L, int... RS>
-__device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile, const float2 *tw,
+__device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile, float2 *tw,
                                                 float2 *xbw, float2 *xs, int wave, int lane) {
     constexpr int V = N / L, H = N;
     const int g = lane / L, l = lane % L;
@@ -452,6 +461,10 @@ __device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile
         const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + off;
 #pragma unroll
         for (int m = 0; m < V; ++m) x[m] = gld2(F + L * m);
+    }
+    // the twiddle table, its loads in flight together with the column's
+    wave_load_twiddles(tw, wd.twH, H, 64 * kMaxKernels);
+    if (wave == 0) {
         wave_fft<N, L, false, 1, 0, RS...>(x, xb, tw, l);
 #pragma unroll
         for (int m = 0; m < V; ++m) xs[m * 64 + lane] = x[m];
@@ -491,12 +504,6 @@ __device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile
 // 24): its registers are sized for them, so a class of small transforms keeps
 // more waves per SIMD.  wave_class() on the host gives a plan's class.
 
-// twiddle tables (n - 1 entries) of the plan -> LDS; every wave waits for them
-__device__ __forceinline__ void wave_load_twiddles(float2 *tw, const float *src, int n, int nThreads) {
-    const LENIA_GLOBAL float2 *g = gp(reinterpret_cast<const float2 *>(src));
-    for (int t = (int)threadIdx.x; t < n - 1; t += nThreads) tw[t] = gld2(g + t);
-    __syncthreads();
-}
 
 // LDS: twiddles [W] | NW exchange regions of M groups
 template <int MODE, bool EXTRAS, int NW, int VC>
@@ -532,15 +539,14 @@ __global__ __launch_bounds__(64 * NW) void k_wave_cols(const FftWorldDev *__rest
     if ((int)blockIdx.x >= nTiles) return;
     const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
     const FftWorldDev &wd = worlds[tile.x];
-    const int H = wd.H;
-    wave_load_twiddles(smem, wd.twH, H, 64 * NW);
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     switch (wd.planH.code) {
 #define LENIA_WAVE_COLS_CASE(code, n, L, ...)                                                   \
     case code:                                                                                   \
         if constexpr (n / L == VC)                                                               \
             wave_cols<n, L, __VA_ARGS__>(wd, tile, smem,                                        \
-                                         smem + n + wave * (64 / L) * wave_group_span<n>(), wave, lane); \
+                                         smem + n + wave * (64 / L) * wave_group_span<n>(), wave, lane, \
+                                         64 * NW);                                               \
         break;
         LENIA_WAVE_PLANS(LENIA_WAVE_COLS_CASE)
 #undef LENIA_WAVE_COLS_CASE
@@ -582,7 +588,6 @@ __global__ __launch_bounds__(64 * kMaxKernels) void k_wave_cols_split(
     if ((int)blockIdx.x >= nTiles) return;
     const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
     const FftWorldDev &wd = worlds[tile.x];
-    wave_load_twiddles(smem, wd.twH, wd.H, 64 * kMaxKernels);
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     switch (wd.planH.code) {
 #define LENIA_WAVE_COLS_SPLIT_CASE(code, n, L, ...)                                        \
