@@ -376,73 +376,83 @@ __device__ __forceinline__ void fft_stage_c(float2 *buf, float2 *dstBuf, int nt,
     }
 }
 
-// stages from src; each stage's destination alternates so the last one ends
-// in buf (alt == buf: all in place)
+// stages from src, returning where the result is.  Each stage's destination
+// alternates between buf and alt (alt == buf: all in place).  anyEnd false:
+// the last stage ends in buf (with an odd stage count the first runs in
+// place); true: every stage moves to the other region and the result ends in
+// whichever the stage count gives.
 template <bool INV, int MAXV, int T, int N, int NS, int TWOFF, int R, int... REST>
-__device__ __forceinline__ void fft_fixed_from(float2 *buf, float2 *alt, float2 *src, int nt,
-                                               const float2 *tw, int tid) {
-    float2 *dst = (sizeof...(REST) & 1) ? alt : buf;
+__device__ __forceinline__ float2 *fft_fixed_from(float2 *buf, float2 *alt, float2 *src, bool anyEnd,
+                                                  int nt, const float2 *tw, int tid) {
+    float2 *dst = anyEnd ? (src == buf ? alt : buf) : ((sizeof...(REST) & 1) ? alt : buf);
     fft_stage_c<R, INV, MAXV, T, N, NS, TWOFF>(src, dst, nt, tw, tid);
     if constexpr (sizeof...(REST) > 0)
-        fft_fixed_from<INV, MAXV, T, N, NS * R, TWOFF + NS * (R - 1), REST...>(buf, alt, dst, nt, tw, tid);
+        return fft_fixed_from<INV, MAXV, T, N, NS * R, TWOFF + NS * (R - 1), REST...>(buf, alt, dst, anyEnd,
+                                                                                     nt, tw, tid);
+    else
+        return dst;
 }
 template <bool INV, int MAXV, int T, int N, int NS, int TWOFF, int R, int... REST>
-__device__ __forceinline__ void fft_fixed(float2 *buf, float2 *alt, int nt, const float2 *tw, int tid) {
-    fft_fixed_from<INV, MAXV, T, N, NS, TWOFF, R, REST...>(buf, alt ? alt : buf, buf, nt, tw, tid);
+__device__ __forceinline__ float2 *fft_fixed(float2 *buf, float2 *alt, bool anyEnd, int nt, const float2 *tw,
+                                             int tid) {
+    return fft_fixed_from<INV, MAXV, T, N, NS, TWOFF, R, REST...>(buf, alt ? alt : buf, buf, anyEnd, nt, tw, tid);
 }
 
 // fft_lds through the compile-time kernel of the plan when it has one
 // the other radix orders of the benchmark lengths (radix-plan tuning), kept
 // out of the default plans' code path
 template <bool INV, int MAXV, int T>
-__device__ __forceinline__ void fft_run_alt(float2 *buf, uint32_t code, int nt,
-                                                      const float2 *tw, int tid, float2 *alt) {
+__device__ __forceinline__ float2 *fft_run_alt(float2 *buf, uint32_t code, int nt,
+                                                         const float2 *tw, int tid, float2 *alt, bool anyEnd) {
+    float2 *out = buf;
     switch (code) {
-        case kPlan64b: fft_fixed<INV, MAXV, T, 64, 1, 0, 16, 4>(buf, alt, nt, tw, tid); break;
-        case kPlan64c: fft_fixed<INV, MAXV, T, 64, 1, 0, 4, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan128b: fft_fixed<INV, MAXV, T, 128, 1, 0, 8, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan128c: fft_fixed<INV, MAXV, T, 128, 1, 0, 4, 4, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan256b: fft_fixed<INV, MAXV, T, 256, 1, 0, 8, 8, 4>(buf, alt, nt, tw, tid); break;
-        case kPlan256c: fft_fixed<INV, MAXV, T, 256, 1, 0, 4, 8, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan512b: fft_fixed<INV, MAXV, T, 512, 1, 0, 16, 8, 4>(buf, alt, nt, tw, tid); break;
-        case kPlan512c: fft_fixed<INV, MAXV, T, 512, 1, 0, 4, 8, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan1024b: fft_fixed<INV, MAXV, T, 1024, 1, 0, 8, 8, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan1024c: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 16, 4>(buf, alt, nt, tw, tid); break;
-        case kPlan1024d: fft_fixed<INV, MAXV, T, 1024, 1, 0, 4, 16, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan2048b: fft_fixed<INV, MAXV, T, 2048, 1, 0, 8, 16, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan4096b: fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan96b: fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, alt, nt, tw, tid); break;
-        case kPlan192b: fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, alt, nt, tw, tid); break;
-        default: fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, alt, nt, tw, tid); break;
+        case kPlan64b: out = fft_fixed<INV, MAXV, T, 64, 1, 0, 16, 4>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan64c: out = fft_fixed<INV, MAXV, T, 64, 1, 0, 4, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan128b: out = fft_fixed<INV, MAXV, T, 128, 1, 0, 8, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan128c: out = fft_fixed<INV, MAXV, T, 128, 1, 0, 4, 4, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan256b: out = fft_fixed<INV, MAXV, T, 256, 1, 0, 8, 8, 4>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan256c: out = fft_fixed<INV, MAXV, T, 256, 1, 0, 4, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan512b: out = fft_fixed<INV, MAXV, T, 512, 1, 0, 16, 8, 4>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan512c: out = fft_fixed<INV, MAXV, T, 512, 1, 0, 4, 8, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1024b: out = fft_fixed<INV, MAXV, T, 1024, 1, 0, 8, 8, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1024c: out = fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 16, 4>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1024d: out = fft_fixed<INV, MAXV, T, 1024, 1, 0, 4, 16, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan2048b: out = fft_fixed<INV, MAXV, T, 2048, 1, 0, 8, 16, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan4096b: out = fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan96b: out = fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan192b: out = fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        default: out = fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, alt, anyEnd, nt, tw, tid); break;
     }
+    return out;
 }
 
 // fft_lds through the compile-time kernel of the plan when it has one
-// alt: an optional second LDS region of the same span (see fft_lds)
+// alt: an optional second LDS region of the same span (see fft_lds).  The
+// result is left in buf, or with anyEnd (and alt) in whichever region the
+// compile-time plans end in; the return value says where.
 template <bool INV, int MAXV, int T>
-__device__ __forceinline__ void fft_run(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
-                                        int tid, float2 *alt = nullptr) {
-    if (__builtin_expect(plan.code >= kPlan64b, 0)) {
-        fft_run_alt<INV, MAXV, T>(buf, plan.code, nt, tw, tid, alt);
-        return;
-    }
+__device__ __forceinline__ float2 *fft_run(float2 *buf, const FftPlan &plan, int nt, const float2 *tw,
+                                           int tid, float2 *alt = nullptr, bool anyEnd = false) {
+    if (__builtin_expect(plan.code >= kPlan64b, 0)) return fft_run_alt<INV, MAXV, T>(buf, plan.code, nt, tw, tid, alt, anyEnd);
+    float2 *out = buf;
     switch (plan.code) {
-        case kPlan64: fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan128: fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan256: fft_fixed<INV, MAXV, T, 256, 1, 0, 16, 16>(buf, alt, nt, tw, tid); break;
-        case kPlan512: fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan1024: fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan2048: fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, alt, nt, tw, tid); break;
-        case kPlan96: fft_fixed<INV, MAXV, T, 96, 1, 0, 8, 4, 3>(buf, alt, nt, tw, tid); break;
-        case kPlan192: fft_fixed<INV, MAXV, T, 192, 1, 0, 8, 8, 3>(buf, alt, nt, tw, tid); break;
-        case kPlan384: fft_fixed<INV, MAXV, T, 384, 1, 0, 16, 8, 3>(buf, alt, nt, tw, tid); break;
-        case kPlan720: fft_fixed<INV, MAXV, T, 720, 1, 0, 16, 3, 3, 5>(buf, alt, nt, tw, tid); break;
-        case kPlan1080: fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, alt, nt, tw, tid); break;
-        case kPlan1280: fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, alt, nt, tw, tid); break;
-        case kPlan1920: fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, alt, nt, tw, tid); break;
-        case kPlan4096: fft_fixed<INV, MAXV, T, 4096, 1, 0, 16, 16, 16>(buf, alt, nt, tw, tid); break;
+        case kPlan64: out = fft_fixed<INV, MAXV, T, 64, 1, 0, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan128: out = fft_fixed<INV, MAXV, T, 128, 1, 0, 16, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan256: out = fft_fixed<INV, MAXV, T, 256, 1, 0, 16, 16>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan512: out = fft_fixed<INV, MAXV, T, 512, 1, 0, 8, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1024: out = fft_fixed<INV, MAXV, T, 1024, 1, 0, 16, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan2048: out = fft_fixed<INV, MAXV, T, 2048, 1, 0, 16, 16, 8>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan96: out = fft_fixed<INV, MAXV, T, 96, 1, 0, 8, 4, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan192: out = fft_fixed<INV, MAXV, T, 192, 1, 0, 8, 8, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan384: out = fft_fixed<INV, MAXV, T, 384, 1, 0, 16, 8, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan720: out = fft_fixed<INV, MAXV, T, 720, 1, 0, 16, 3, 3, 5>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1080: out = fft_fixed<INV, MAXV, T, 1080, 1, 0, 8, 3, 3, 3, 5>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1280: out = fft_fixed<INV, MAXV, T, 1280, 1, 0, 16, 16, 5>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan1920: out = fft_fixed<INV, MAXV, T, 1920, 1, 0, 16, 8, 3, 5>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan4096: out = fft_fixed<INV, MAXV, T, 4096, 1, 0, 16, 16, 16>(buf, alt, anyEnd, nt, tw, tid); break;
         default: fft_lds<INV, MAXV, T>(buf, plan, nt, tw, tid, alt); break;
     }
+    return out;
 }
 
 // XCD-aware tile order: consecutive workgroup ids land on different XCDs

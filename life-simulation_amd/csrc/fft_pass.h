@@ -137,13 +137,14 @@ __device__ __forceinline__ void forward_rows(float2 *Z, float2 *Zt, const FftWor
                                              const float2 *tw, int rp0, int nt, int tid) {
     const int W = wd.W, H2 = wd.H / 2, P = wd.P;
     const int nm = P * wd.Wh;
-    fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid, Zt);
+    // the spectra may end in either region
+    const float2 *Zs = fft_run<false, kMaxV, T>(Z, wd.planW, nt, tw, tid, Zt, true);
     LENIA_GLOBAL float4 *F4 = gp(reinterpret_cast<float4 *>(wd.F)) + rp0;
     for (int m = tid; m < nm; m += T) {
         int kx = (int)fast_div((uint32_t)m, wd.mP), p = m - kx * P;
         if (p >= nt) continue;
-        float2 z1 = Z[sw(p * W + kx)];
-        float2 z2 = Z[sw(p * W + (kx == 0 ? 0 : W - kx))];
+        float2 z1 = Zs[sw(p * W + kx)];
+        float2 z2 = Zs[sw(p * W + (kx == 0 ? 0 : W - kx))];
         gst4(F4 + kx * H2 + p, make_float4(0.5f * (z1.x + z2.x), 0.5f * (z1.y - z2.y),
                                            0.5f * (z1.y + z2.y), 0.5f * (z2.x - z1.x)));
     }
@@ -222,12 +223,13 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         if (newWorld) twc.store(tw, twg, W, tid);
         gather_store<kG>(Z, gv, zi, kxv, W);
         __syncthreads();
-        fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid, Zt);
+        // the potentials may end in either region (U); the new state goes to Z
+        const float2 *U = fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid, Zt, true);
         if (MODE == 2) {
             for (int q = tid; q < nq; q += T) {
                 int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
                 int ia = 2 * p * W + x;
-                float2 z = Z[sw(q)];
+                float2 z = U[sw(q)];
                 LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
                 o[ia] = z.x;
                 o[ia + W] = z.y;
@@ -249,7 +251,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
                 float2 z[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    if (k < kn) z[k] = Z[sw(k * nq + q)];
+                    if (k < kn) z[k] = U[sw(k * nq + q)];
                 float sa = 0.0f, sb = 0.0f;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
