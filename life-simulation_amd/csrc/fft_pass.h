@@ -190,8 +190,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         }
         __syncthreads();
         forward_rows<T>(Z, Zt, wd, tw, rp0, nt, tid);
-        __syncthreads();
-        return;
+        return;  // one tile per workgroup: no LDS reuse after it
     }
     const GrowthParams &g = wd.g;
     const GrowthRegs gr = growth_regs(g);
@@ -234,7 +233,6 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
                 o[ia] = z.x;
                 o[ia + W] = z.y;
             }
-            __syncthreads();
             return;
         }
 #pragma unroll
@@ -359,7 +357,6 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     }
     __syncthreads();
     forward_rows<T>(Z, Zt, wd, tw, rp0, nt, tid);
-    __syncthreads();
 }
 
 // One tile per workgroup (grid = tiles)
@@ -453,7 +450,6 @@ __device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds
             int k = (e >= ne) + (e >= 2 * ne) + (e >= 3 * ne), c = e - k * ne;
             gst2(G0 + (k * (int)sStride + c), Y[sw(e)]);
         }
-        __syncthreads();
         return;
     }
     float sr[V];
@@ -496,7 +492,7 @@ __device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds
         fft_lds<true, kMaxV, T>(X, wd.planH, nt, tw, tid);
         LENIA_GLOBAL float2 *G = G0 + (size_t)k * sStride;
         for (int e = tid; e < ne; e += T) gst2(G + e, X[sw(e)]);
-        __syncthreads();
+        if (k + 1 < K) __syncthreads();  // X is refilled for the next kernel
     }
 }
 
