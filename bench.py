@@ -369,7 +369,7 @@ def matrix_bench(ctx, cpu_s: float, tune_plans: bool = True) -> dict:
     with lenia.World(ctx, d) as w:
         w.reset(7777)
         w.frame(0, 1)
-        w.step(10)
+        w.step(64)  # also records the world's step graph, outside the timed steps
         ctx.sync()
         w.step(200)
         ms = ctx.last_gpu_ms()
@@ -397,7 +397,7 @@ def matrix_bench(ctx, cpu_s: float, tune_plans: bool = True) -> dict:
             w.reset(7777)
             w.frame(0, 1)
             w.autotune(10)
-            w.step(20)
+            w.step(64)  # also records the world's step graph, outside the timed steps
             ctx.sync()
             w.step(500)
             ms = ctx.last_gpu_ms()

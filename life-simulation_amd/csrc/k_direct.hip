@@ -5,17 +5,23 @@
 // extended to up to four kernels with the leniaGrowthMulti rule
 // (Lenia.metal:161-232).
 //
-// A 256-thread workgroup owns a 64 (x) x 32 (y) output tile and stages its
-// (32+2h) x (64+P) toroidal halo in LDS once (P = window width rounded up to
-// 8; the window rows are zero padded to P).  Each thread produces 8
-// horizontally adjacent outputs of one row.  For every window row it slides
+// A workgroup owns a 64 (x) x 32 (y) output tile and stages its (32+2h) x
+// (64+P) toroidal halo in LDS once (P = window width rounded up to 8; the
+// window rows are zero padded to P).  Each output's potential is the sum over
+// the first ceil(n/2) window rows plus the sum over the others.  In the
+// 256-thread form one thread produces 8 horizontally adjacent outputs of one
+// row, both halves in turn; in the 512-thread form (SPLIT, for grids of at
+// most one tile per CU) a thread of the first half of the
+// workgroup sums the first window-row half and its partner in the second
+// half the other, and the first adds the partner's sums (through LDS).  Both
+// forms compute the same bits.  For every window row it slides
 // a 16-value register window along the staged row, 8 taps at a time: two
 // 16-byte LDS reads of state and two of weights (the kernel's window is
 // staged in LDS too) feed 64 FMAs, with the next chunk's reads issued before
 // the current chunk's FMAs.
-// Each output's sum is taken per window row first, then across rows
-// (two-level summation keeps the FP32 potential within ~1e-6 of the FP64
-// value for 3000-tap windows).
+// Each output's sum is taken per window row first, then across rows of each
+// half, then the two halves (this summation keeps the FP32 potential within
+// ~1e-6 of the FP64 value for 3000-tap windows).
 #include <hip/hip_runtime.h>
 
 #include "lenia_device.h"
@@ -23,7 +29,7 @@
 
 namespace lenia {
 
-constexpr int kDT = 256;   // threads: 8 (x) x 32 (y)
+constexpr int kDHalf = 256;  // threads of one window-row half: 8 (x) x 32 (y)
 constexpr int kDTX = 64;   // output tile width
 constexpr int kDTY = 32;   // output tile height
 constexpr int kDOX = 8;    // outputs per thread (x)
@@ -61,15 +67,17 @@ __device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const
 }
 
 // MODE 0: full step (src -> dst by clock parity); MODE 1: potential of kernel
-// kSel of the current state into `out`
-template <int MODE>
-__global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict__ worlds,
-                                                const int2 *__restrict__ tiles, int nTiles,
-                                                uint32_t stepOff, int kSel,
-                                                float *__restrict__ out) {
+// kSel of the current state into `out`.  SPLIT: 512 threads, one window-row
+// half each; else 256 threads, both halves each.
+template <int MODE, bool SPLIT>
+__global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
+    const DirectWorldDev *__restrict__ worlds, const int2 *__restrict__ tiles, int nTiles, uint32_t stepOff,
+    int kSel, float *__restrict__ out) {
+    constexpr int kDT = SPLIT ? 2 * kDHalf : kDHalf;
     extern __shared__ float tileS[];
     const int tid = threadIdx.x;
-    const int tx = tid & 7, ty = tid >> 3;
+    const int half = SPLIT && tid >= kDHalf, ht = tid - half * kDHalf;  // window-row half, thread in it
+    const int tx = ht & 7, ty = ht >> 3;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
         const int2 tile = tiles[ti];
         const DirectWorldDev &wd = worlds[tile.x];
@@ -113,6 +121,8 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
         const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
         const int pitch4 = pitch / 4;
         float *winS = tileS + rows * pitch;  // the current kernel's window, [n][np]
+        float4 *part4 = reinterpret_cast<float4 *>(winS + n * np);  // second-half sums [2][256]
+        const int nh = (n + 1) / 2;
         for (int k = kBeg; k < kEnd; ++k) {
             // stage kernel k's window in LDS: every load of the tap loop is
             // then an LDS load, and those complete in order, so the loop can
@@ -125,47 +135,77 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
                 for (int i = tid; i < n * (np / 4); i += kDT) ws4[i] = gld4(wg + i);
             }
             __syncthreads();
+            const LENIA_CONST int *rc = (const LENIA_CONST int *)(wd.rowChunks) + (size_t)2 * k * n;
+            // the sum of window rows iy0 .. iy1-1 into tot
+            auto rows_sum = [&](int iy0, int iy1, float (&tot)[kDOX]) {
+#pragma unroll
+                for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
+                for (int iy = iy0; iy < iy1; ++iy) {
+                    const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
+                    const float4 *wrow = reinterpret_cast<const float4 *>(winS) + iy * (np / 4);
+                    // chunks [c0, c1) of this row hold its non-zero weights
+                    const int c1 = rc[2 * iy + 1];
+                    float rs[kDOX];
+#pragma unroll
+                    for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
+                    // va, vb: state chunks c, c+1; w0: weights of chunk c.  Each
+                    // iteration loads chunk c+2 and its weights before the taps of
+                    // chunk c, and chunk c+3 and the weights of c+2 before the
+                    // taps of chunk c+1 (reads stay inside the row: chunk c+3 is
+                    // loaded only when c+3 <= c1)
+                    float va[8], vb[8], w0[8];
+                    int c = rc[2 * iy];
+                    lds_read8(trow + 2 * c, va);
+                    lds_read8(trow + 2 * c + 2, vb);
+                    lds_read8(wrow + 2 * c, w0);
+                    for (; c + 1 < c1; c += 2) {
+                        float vc[8], w1[8];
+                        lds_read8(trow + 2 * c + 4, vc);
+                        lds_read8(wrow + 2 * c + 2, w1);
+                        taps8(rs, w0, va, vb);
+                        if (c + 3 <= c1) lds_read8(trow + 2 * c + 6, va);
+                        if (c + 2 < c1) lds_read8(wrow + 2 * c + 4, w0);
+                        taps8(rs, w1, vb, vc);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const float t = va[i];
+                            va[i] = vc[i];
+                            vb[i] = t;
+                        }
+                    }
+                    if (c < c1) taps8(rs, w0, va, vb);
+#pragma unroll
+                    for (int i = 0; i < kDOX; ++i) tot[i] += rs[i];
+                }
+            };
+            float lo[kDOX], hi[kDOX];
+            if (SPLIT) {
+                // the first half adds its partner's sums
+                if (half) {
+                    rows_sum(nh, n, hi);
+                    part4[ht] = make_float4(hi[0], hi[1], hi[2], hi[3]);
+                    part4[kDHalf + ht] = make_float4(hi[4], hi[5], hi[6], hi[7]);
+                } else {
+                    rows_sum(0, nh, lo);
+                }
+                __syncthreads();
+                if (half) continue;
+                const float4 a = part4[ht], b = part4[kDHalf + ht];
+                hi[0] = a.x;
+                hi[1] = a.y;
+                hi[2] = a.z;
+                hi[3] = a.w;
+                hi[4] = b.x;
+                hi[5] = b.y;
+                hi[6] = b.z;
+                hi[7] = b.w;
+            } else {
+                rows_sum(0, nh, lo);
+                rows_sum(nh, n, hi);
+            }
             float tot[kDOX];
 #pragma unroll
-            for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
-            const LENIA_CONST int *rc = (const LENIA_CONST int *)(wd.rowChunks) + (size_t)2 * k * n;
-            for (int iy = 0; iy < n; ++iy) {
-                const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
-                const float4 *wrow = reinterpret_cast<const float4 *>(winS) + iy * (np / 4);
-                // chunks [c0, c1) of this row hold its non-zero weights
-                const int c1 = rc[2 * iy + 1];
-                float rs[kDOX];
-#pragma unroll
-                for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
-                // va, vb: state chunks c, c+1; w0: weights of chunk c.  Each
-                // iteration loads chunk c+2 and its weights before the taps of
-                // chunk c, and chunk c+3 and the weights of c+2 before the
-                // taps of chunk c+1 (reads stay inside the row: chunk c+3 is
-                // loaded only when c+3 <= c1)
-                float va[8], vb[8], w0[8];
-                int c = rc[2 * iy];
-                lds_read8(trow + 2 * c, va);
-                lds_read8(trow + 2 * c + 2, vb);
-                lds_read8(wrow + 2 * c, w0);
-                for (; c + 1 < c1; c += 2) {
-                    float vc[8], w1[8];
-                    lds_read8(trow + 2 * c + 4, vc);
-                    lds_read8(wrow + 2 * c + 2, w1);
-                    taps8(rs, w0, va, vb);
-                    if (c + 3 <= c1) lds_read8(trow + 2 * c + 6, va);
-                    if (c + 2 < c1) lds_read8(wrow + 2 * c + 4, w0);
-                    taps8(rs, w1, vb, vc);
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const float t = va[i];
-                        va[i] = vc[i];
-                        vb[i] = t;
-                    }
-                }
-                if (c < c1) taps8(rs, w0, va, vb);
-#pragma unroll
-                for (int i = 0; i < kDOX; ++i) tot[i] += rs[i];
-            }
+            for (int i = 0; i < kDOX; ++i) tot[i] = lo[i] + hi[i];
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) {
                 const int x = xb + i;
@@ -177,7 +217,7 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
                 }
             }
         }
-        if (MODE == 0) {
+        if (MODE == 0 && !half) {
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) {
                 const int x = xb + i;
@@ -191,15 +231,26 @@ __global__ __launch_bounds__(kDT) void k_direct(const DirectWorldDev *__restrict
     }
 }
 
+template <bool SPLIT>
+static void direct_launch(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
+                          size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s) {
+    constexpr int T = SPLIT ? 2 * kDHalf : kDHalf;
+    if (mode == 0)
+        launch_timed(k_direct<0, SPLIT>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel, out);
+    else
+        launch_timed(k_direct<1, SPLIT>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel, out);
+}
+
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
                          int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                         hipStream_t s) {
-    if (mode == 0)
-        launch_timed(k_direct<0>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, stepOff,
-                     kSel, out);
+                         hipStream_t s, int numCUs) {
+    // grids that leave CUs without a workgroup take the 512-thread form
+    // (bit-identical); from one tile per CU on, the 256-thread form, which
+    // keeps two workgroups per CU (1280x720 R=32: 460 tiles, 113 against 124 us)
+    if (nTiles <= numCUs)
+        direct_launch<true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     else
-        launch_timed(k_direct<1>, dim3(grid), dim3(kDT), lds, s, worlds, tiles, nTiles, stepOff,
-                     kSel, out);
+        direct_launch<false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     return hipGetLastError();
 }
 
@@ -213,8 +264,9 @@ int direct_npad(int h) { return (2 * h + 1 + 7) & ~7; }
 
 size_t direct_lds_bytes(int h) {
     const int pitch = kDTX + direct_npad(h) + 4;
-    // the halo tile, then one kernel window [2h+1][npad]
-    return ((size_t)(kDTY + 2 * h) * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float);
+    // the halo tile, one kernel window [2h+1][npad], the second half's sums
+    return ((size_t)(kDTY + 2 * h) * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float) +
+           (size_t)2 * kDHalf * sizeof(float4);
 }
 
 }  // namespace lenia

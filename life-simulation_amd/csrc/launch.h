@@ -70,9 +70,11 @@ int fft_max_values();
 int fft_row_values(int T);
 
 // direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.
+// grids of at most numCUs tiles run the 512-thread form (each output's two
+// window-row halves on two threads; the same bits)
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
                          int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                         hipStream_t s);
+                         hipStream_t s, int numCUs);
 void direct_tile_shape(int *tx, int *ty);
 int direct_npad(int h);
 size_t direct_lds_bytes(int h);

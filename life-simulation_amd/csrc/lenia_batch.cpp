@@ -366,7 +366,7 @@ static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint3
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                          b->nRowTiles, b->ldsRows, off, 0, nullptr, s);
+                          b->nRowTiles, b->ldsRows, off, 0, nullptr, s, b->ctx->numCUs);
         launch_events() = LaunchEvents();
     }
     return e;

@@ -1091,7 +1091,7 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
+                          w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream, ctx->numCUs);
     }
     launch_events() = LaunchEvents();
     return e;
@@ -1310,7 +1310,7 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
                         w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
     } else {
         e = launch_direct(1, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
+                          w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream, ctx->numCUs);
     }
     if (e != hipSuccess) return hip_fail(e, "potential launch");
     HIP_TRY(hipMemcpyAsync(host, w->scratchOut, cells * sizeof(float), hipMemcpyDeviceToHost,
