@@ -116,6 +116,20 @@ def test_direct_batch_bit_exact_vs_alone(ctx):
         assert got[i].tobytes() == run_alone(ctx, d, seeds[i], 17).tobytes(), i
 
 
+def test_direct_batch_of_many_tiles_bit_exact_vs_alone(ctx):
+    """40 direct worlds of 128x128 (8 output tiles each): the batch's 320
+    tiles run the 256-thread form of the FP32 direct kernel, each world alone
+    (8 tiles) the 512-thread form that splits each output's window rows over
+    two threads; both must give the same bytes."""
+    descs = [dict(width=128, height=128, convMode=DIRECT, growthMode=i % 2, dt=0.1, radius=9 + i % 5,
+                  kernels=[ring(9 + i % 5, [1, 0.5][: 1 + i % 2], 0.15 + 0.002 * i, 0.02, 1.0)],
+                  initMode=0, initRect=(32, 32, 64, 64)) for i in range(40)]
+    seeds = [300 + i for i in range(40)]
+    got = run_batched(ctx, descs, seeds, 9, 0)
+    for i in range(0, 40, 3):
+        assert got[i].tobytes() == run_alone(ctx, descs[i], seeds[i], 9).tobytes(), i
+
+
 def test_batch_with_extras_bit_exact(ctx):
     base = dict(width=96, height=64, convMode=FFT, growthMode=1, dt=0.35, radius=8,
                 noiseAmount=0.4, muJitter=0.5, injectAmount=0.3, injectCount=3,
