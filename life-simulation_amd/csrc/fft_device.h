@@ -421,6 +421,7 @@ __device__ __forceinline__ float2 *fft_run_alt(float2 *buf, uint32_t code, int n
         case kPlan4096b: out = fft_fixed<INV, MAXV, T, 4096, 1, 0, 8, 8, 8, 8>(buf, alt, anyEnd, nt, tw, tid); break;
         case kPlan96b: out = fft_fixed<INV, MAXV, T, 96, 1, 0, 4, 8, 3>(buf, alt, anyEnd, nt, tw, tid); break;
         case kPlan192b: out = fft_fixed<INV, MAXV, T, 192, 1, 0, 16, 4, 3>(buf, alt, anyEnd, nt, tw, tid); break;
+        case kPlan384c: out = fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 4, 4, 3>(buf, alt, anyEnd, nt, tw, tid); break;
         default: out = fft_fixed<INV, MAXV, T, 384, 1, 0, 8, 16, 3>(buf, alt, anyEnd, nt, tw, tid); break;
     }
     return out;

@@ -498,7 +498,8 @@ __device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile
     X(kPlan512, 512, 64, 8, 8, 8)           \
     X(kPlan1024, 1024, 64, 16, 8, 8)        \
     X(kPlan96, 96, 4, 8, 4, 3)              \
-    X(kPlan192, 192, 8, 8, 8, 3)
+    X(kPlan192, 192, 8, 8, 8, 3)            \
+    X(kPlan384c, 384, 16, 8, 4, 4, 3)
 
 // A kernel runs the plans of one values-per-lane class VC = N / L (8, 16 or
 // 24): its registers are sized for them, so a class of small transforms keeps

@@ -522,33 +522,59 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
     // the fastest whole step is kept.
     double graphStepMs = 0.0;
     if (!rc && bestRowT && bestColT) {
+        std::vector<uint64_t> hostSteps;
+        for (lenia_world *w : b->worlds) hostSteps.push_back(w->hostSteps);
+        // graph-replayed time per step of the batch as currently configured
+        // (negative: the configuration does not run)
+        auto graph_step = [&]() -> double {
+            constexpr uint32_t kG = 8, kRuns = 4;
+            if (batch_setup(b) != LENIA_OK) return -1.0;
+            if (batch_run_steps(b, kG, kG) != LENIA_OK) return -1.0;  // capture, warm-up
+            if (batch_run_steps(b, kRuns * kG, kG) != LENIA_OK) return -1.0;
+            float ms = 0.f;
+            if (hipEventSynchronize(ctx->ev1) != hipSuccess ||
+                hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess)
+                return -1.0;
+            return ms / (kRuns * kG);
+        };
         std::vector<std::pair<int, int>> cand = {{bestRowNW, bestColNW}, {0, 0}};
         for (int ni = 0; ni < 4; ++ni) cand.push_back({bestRowNW, kNw[ni]});
         double bestStep = 1e30;
         std::pair<int, int> pick = cand[0];
-        std::vector<uint64_t> hostSteps;
-        for (lenia_world *w : b->worlds) hostSteps.push_back(w->hostSteps);
-        constexpr uint32_t kG = 8, kRuns = 4;
         for (size_t i = 0; i < cand.size(); ++i) {
             if (std::find(cand.begin(), cand.begin() + i, cand[i]) != cand.begin() + i) continue;
             b->rowWaveNW = cand[i].first;
             b->colWaveNW = cand[i].second;
-            if (batch_setup(b) != LENIA_OK) continue;
-            if (batch_run_steps(b, kG, kG) != LENIA_OK) continue;  // capture, warm-up
-            if (batch_run_steps(b, kRuns * kG, kG) != LENIA_OK) continue;
-            float ms = 0.f;
-            if (hipEventSynchronize(ctx->ev1) != hipSuccess ||
-                hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess)
-                continue;
-            const double per = ms / (kRuns * kG);
-            if (per < bestStep) {
+            const double per = graph_step();
+            if (per > 0.0 && per < bestStep) {
                 bestStep = per;
                 pick = cand[i];
             }
         }
-        for (size_t i = 0; i < b->worlds.size(); ++i) b->worlds[i]->hostSteps = hostSteps[i];
         bestRowNW = pick.first;
         bestColNW = pick.second;
+        // with wave columns the LDS shape of the column pass serves only the
+        // worlds without a wave plan, whose launch runs beside the wave
+        // classes: its shape is chosen again for that group, the same way
+        if (bestColNW && bestStep < 1e30) {
+            b->rowWaveNW = bestRowNW;
+            b->colWaveNW = bestColNW;
+            for (int kp = 1; kp >= 0; --kp)
+                for (int T = 64; T <= 512; T *= 2) {
+                    if (T == bestColT && (kp != 0) == bestColKp) continue;
+                    b->tuneColT = T;
+                    b->tuneColKpar = kp != 0;
+                    const double per = graph_step();
+                    if (per > 0.0 && per < bestStep) {
+                        bestStep = per;
+                        bestColT = T;
+                        bestColKp = kp != 0;
+                    }
+                }
+            b->tuneColT = bestColT;
+            b->tuneColKpar = bestColKp;
+        }
+        for (size_t i = 0; i < b->worlds.size(); ++i) b->worlds[i]->hostSteps = hostSteps[i];
         if (bestStep < 1e30) graphStepMs = bestStep;
     }
     b->rowWaveNW = bestRowNW;

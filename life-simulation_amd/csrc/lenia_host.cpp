@@ -236,7 +236,8 @@ const Fixed kFixedPlans[] = {
     {1024, 3, {8, 8, 16}, kPlan1024b}, {1024, 3, {16, 16, 4}, kPlan1024c},
     {1024, 3, {4, 16, 16}, kPlan1024d}, {2048, 3, {8, 16, 16}, kPlan2048b},
     {4096, 4, {8, 8, 8, 8}, kPlan4096b}, {96, 3, {4, 8, 3}, kPlan96b},
-    {192, 3, {16, 4, 3}, kPlan192b},   {384, 3, {8, 16, 3}, kPlan384b}};
+    {192, 3, {16, 4, 3}, kPlan192b},   {384, 3, {8, 16, 3}, kPlan384b},
+    {384, 4, {8, 4, 4, 3}, kPlan384c}};
 }  // namespace
 
 // fills the per-stage constants of a radix sequence; false if the product is

@@ -113,6 +113,7 @@ enum : uint32_t {
     kPlan96b = 28,    // 4 8 3
     kPlan192b = 29,   // 16 4 3
     kPlan384b = 30,   // 8 16 3
+    kPlan384c = 31,   // 8 4 4 3: every radix divides 24, so 384 runs on the wave executor
 };
 
 struct FftPlan {

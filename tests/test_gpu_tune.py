@@ -139,7 +139,7 @@ COMPILED_PLANS = {
     64: [0x88, 0x4F, 0xF4], 128: [0x8F, 0xF8, 0x844], 256: [0xFF, 0x488, 0x884],
     512: [0x888, 0x48F, 0xF84], 1024: [0x88F, 0xF88, 0x4FF, 0xFF4],
     2048: [0x8FF, 0xFF8], 4096: [0xFFF, 0x8888], 96: [0x348, 0x384], 192: [0x388, 0x34F],
-    384: [0x38F, 0x3F8],
+    384: [0x38F, 0x3F8, 0x3448],
 }
 
 

@@ -72,6 +72,24 @@ def test_wave_equals_lds_executor(ctx, W, H, K):
     assert live > 0
 
 
+# plans with a wave kernel that are not a length's default: (W, H, planW, planH)
+EXPLICIT = [(384, 192, 0x3448, 0), (128, 384, 0, 0x3448)]
+
+
+@pytest.mark.parametrize("W,H,pw,ph", EXPLICIT)
+@pytest.mark.parametrize("K", [1, 3])
+def test_wave_equals_lds_executor_explicit_plans(ctx, W, H, pw, ph, K):
+    d = desc(W, H, K, mode=K % 2, fftPlanW=pw, fftPlanH=ph)
+    a, ra = run(ctx, dict(d, rowEngine=LDS, colEngine=LDS), potentials=True)
+    for eng in (WAVE, SPLIT):
+        b, rb = run(ctx, dict(d, rowEngine=eng, colEngine=eng), potentials=True)
+        assert (rb["rowEngine"], rb["colEngine"]) == (eng, eng)
+        assert (ra["fftPlanW"], ra["fftPlanH"]) == (rb["fftPlanW"], rb["fftPlanH"])
+        for x, y in zip(a, b):
+            assert x.tobytes() == y.tobytes(), eng
+    assert int(np.count_nonzero((a[0] > 0) & (a[0] < 1))) > 0
+
+
 @pytest.mark.parametrize("W,H", [(512, 192), (96, 256)])
 def test_wave_extras_and_mixed_executors(ctx, W, H):
     """noise / jitter / inject terms on the wave executor, and one pass on
