@@ -553,27 +553,6 @@ extern "C" int lenia_batch_autotune(lenia_batch *b, uint32_t nSteps, double *msP
         }
         bestRowNW = pick.first;
         bestColNW = pick.second;
-        // with wave columns the LDS shape of the column pass serves only the
-        // worlds without a wave plan, whose launch runs beside the wave
-        // classes: its shape is chosen again for that group, the same way
-        if (bestColNW && bestStep < 1e30) {
-            b->rowWaveNW = bestRowNW;
-            b->colWaveNW = bestColNW;
-            for (int kp = 1; kp >= 0; --kp)
-                for (int T = 64; T <= 512; T *= 2) {
-                    if (T == bestColT && (kp != 0) == bestColKp) continue;
-                    b->tuneColT = T;
-                    b->tuneColKpar = kp != 0;
-                    const double per = graph_step();
-                    if (per > 0.0 && per < bestStep) {
-                        bestStep = per;
-                        bestColT = T;
-                        bestColKp = kp != 0;
-                    }
-                }
-            b->tuneColT = bestColT;
-            b->tuneColKpar = bestColKp;
-        }
         for (size_t i = 0; i < b->worlds.size(); ++i) b->worlds[i]->hostSteps = hostSteps[i];
         if (bestStep < 1e30) graphStepMs = bestStep;
     }
