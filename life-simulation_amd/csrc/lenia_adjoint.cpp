@@ -218,14 +218,16 @@ static int ws_prepare(lenia_world *w, uint32_t T, uint32_t c) {
             return rc;
         }
         a.D = ws->D;
-        // workgroup sizes: about 2 values per thread for the K transforms of
+        // workgroup sizes: about 1 value per thread for the K transforms of
         // a row pair or column -- short stages over many threads, since one
-        // reverse step has only H/2 + W/2+1 workgroups -- within <= 16 values
-        // per thread and the workgroup's LDS
+        // reverse step has only H/2 + W/2+1 workgroups (256x256, K = 2: 512
+        // threads, reverse pass 1.46 -> 1.39 ms against 256; 1024 gave the
+        // same, 128 was slower) -- within <= 16 values per thread and the
+        // workgroup's LDS
         const int mv = fft_max_values();
         auto pick = [&](const FftPlan &p, int nt, bool rows) {
             int want = 64;
-            while (want < 512 && want * 2 < nt * p.n) want *= 2;
+            while (want < 512 && want < nt * p.n) want *= 2;
             for (int T2 = want; T2 <= 512; T2 *= 2)
                 if (plan_fits(p, nt, T2, mv) && plan_fits(p, 1, T2, mv) &&
                     (rows ? adj_lds_rows(a, T2) : adj_lds_cols(a, T2)) <= kFftLdsMax)
