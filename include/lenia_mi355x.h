@@ -229,7 +229,7 @@ int lenia_plan_cache_load(const char *path);
  * frameIndex + i, substep 0, with module seed deriveSeed(sceneSeed, 'LEN1'):
  * the results equal lenia_world_step's on a world of the same descriptor,
  * reset with sceneSeed, holding the same state at that frame.  Spectra,
- * twiddles, tile lists and the device record live in a library-owned device
+ * twiddles and the device record live in a library-owned device
  * array of 64 MiB per GPU (taken when the first context on a GPU is created,
  * held until the process ends); no call allocates.  Fields whose buffers do
  * not fit it (a 1920x1080 field with 3 kernels fits) are rejected with

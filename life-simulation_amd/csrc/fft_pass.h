@@ -367,7 +367,7 @@ __global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ 
                                                 float *__restrict__ out) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x < nTiles) {
-        const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
+        const int2 tile = step_tile(tiles, tile_of((int)blockIdx.x, nTiles));
         rows_tile<MODE, T, EXTRAS, KPAR, V>(worlds, tile, true, stepOff, kSel, out, smem,
                                             (int)threadIdx.x);
     }
@@ -502,7 +502,7 @@ __global__ __launch_bounds__(T) void k_fft_cols(const FftWorldDev *__restrict__ 
                                                 const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x < nTiles) {
-        const int2 tile = tiles[tile_of((int)blockIdx.x, nTiles)];
+        const int2 tile = step_tile(tiles, tile_of((int)blockIdx.x, nTiles));
         cols_tile<T, KPAR, V>(worlds, tile, true, smem, (int)threadIdx.x);
     }
 }

@@ -514,7 +514,7 @@ __global__ __launch_bounds__(64 * NW) void k_wave_rows(const FftWorldDev *__rest
                                                        float *__restrict__ out) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x >= nTiles) return;
-    const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
+    const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     const int W = wd.W;
     wave_load_twiddles(smem, wd.twW, W, 64 * NW);
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(64 * NW) void k_wave_cols(const FftWorldDev *__rest
                                                        const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x >= nTiles) return;
-    const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
+    const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     switch (wd.planH.code) {
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(64 * kMaxKernels) void k_wave_rows_split(
     const FftWorldDev *__restrict__ worlds, const int2 *__restrict__ tiles, int nTiles, uint32_t stepOff) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x >= nTiles) return;
-    const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
+    const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     wave_load_twiddles(smem, wd.twW, wd.W, 64 * kMaxKernels);
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(64 * kMaxKernels) void k_wave_cols_split(
     const FftWorldDev *__restrict__ worlds, const int2 *__restrict__ tiles, int nTiles) {
     extern __shared__ float2 smem[];
     if ((int)blockIdx.x >= nTiles) return;
-    const int2 tile = tiles[xcd_tile((int)blockIdx.x, nTiles)];
+    const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
     switch (wd.planH.code) {

@@ -774,13 +774,6 @@ int lenia::world_upload_desc(lenia_world *w) {
         const FftWorldDev f = fft_record(w);
         HIP_TRY(hipMemcpyAsync(w->devDesc, &f, sizeof(f), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
-        std::vector<int2> rt, ct;
-        for (int i = 0; i < f.rowTiles; ++i) rt.push_back(make_int2(0, i));
-        for (int i = 0; i < f.colTiles; ++i) ct.push_back(make_int2(0, i));
-        HIP_TRY(hipMemcpyAsync(w->rowTiles, rt.data(), rt.size() * sizeof(int2),
-                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
-        HIP_TRY(hipMemcpyAsync(w->colTiles, ct.data(), ct.size() * sizeof(int2),
-                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
     } else if (w->isMfma) {
         MfmaWorldDev md;
         std::memset(&md, 0, sizeof(md));
@@ -904,10 +897,8 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         if (!rc) rc = dev_alloc(ctx, (void **)&w->S, K * Wh * H * sizeof(float));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twW, (size_t)W * sizeof(float2));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twH, (size_t)H * sizeof(float2));
-        // sized for one transform per tile, the most tiles any shape uses
-        int nr = H / 2, nc = Wh;
-        if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nr * sizeof(int2));
-        if (!rc) rc = dev_alloc(ctx, (void **)&w->colTiles, nc * sizeof(int2));
+        // no tile tables: the step kernels take tile j of world 0 from the
+        // workgroup id (step_tile)
     } else {
         int tx, ty;
         if (w->isMfma)
@@ -1037,26 +1028,30 @@ int lenia::world_apply_init(lenia_world *w) {
 hipError_t lenia::world_launch_rows(lenia_world *w, int mode, const FftWorldDev *rec, uint32_t off,
                                     int kSel, float *out, bool extras, hipStream_t s) {
     extras = extras && mode == 1;
+    // no tile table: a single world's tiles are 0, 1, ... of world 0, which
+    // the kernels derive from the workgroup id (step_tile)
+    const int2 *tiles = nullptr;
     if (w->rowWave) {
         const int vc = wave_class(w->planW.code, w->planW.n);
         if (w->rowSplit && mode == 1)
-            return launch_wave_rows_split(extras, vc, rec, w->rowTiles, w->nRowTiles, w->ldsRows, off, s);
-        return launch_wave_rows(mode, w->rowSplit ? 1 : w->rowNW, vc, extras, rec, w->rowTiles,
+            return launch_wave_rows_split(extras, vc, rec, tiles, w->nRowTiles, w->ldsRows, off, s);
+        return launch_wave_rows(mode, w->rowSplit ? 1 : w->rowNW, vc, extras, rec, tiles,
                                 w->nRowTiles, w->ldsRows, off, kSel, out, s);
     }
     // forward-only passes run the serial form, potentials the side-by-side one
     const bool kpar = mode == 1 ? w->rowKpar : mode == 2;
-    return launch_fft_rows(mode, w->rowT, extras, kpar, rec, w->rowTiles, w->nRowTiles, w->nRowTiles,
+    return launch_fft_rows(mode, w->rowT, extras, kpar, rec, tiles, w->nRowTiles, w->nRowTiles,
                            w->ldsRows, off, kSel, out, s, mode == 1 ? w->row_tile_cells() : 0);
 }
 
 hipError_t lenia::world_launch_cols(lenia_world *w, const FftWorldDev *rec, hipStream_t s) {
+    const int2 *tiles = nullptr;  // tiles 0, 1, ... of world 0 (step_tile)
     if (w->colWave) {
         const int vc = wave_class(w->planH.code, w->planH.n);
-        if (w->colSplit) return launch_wave_cols_split(vc, rec, w->colTiles, w->nColTiles, w->ldsCols, s);
-        return launch_wave_cols(w->colNW, vc, rec, w->colTiles, w->nColTiles, w->ldsCols, s);
+        if (w->colSplit) return launch_wave_cols_split(vc, rec, tiles, w->nColTiles, w->ldsCols, s);
+        return launch_wave_cols(w->colNW, vc, rec, tiles, w->nColTiles, w->ldsCols, s);
     }
-    return launch_fft_cols(w->colT, w->colKpar, rec, w->colTiles, w->nColTiles, w->nColTiles,
+    return launch_fft_cols(w->colT, w->colKpar, rec, tiles, w->nColTiles, w->nColTiles,
                            w->ldsCols, s, w->col_tile_cells());
 }
 

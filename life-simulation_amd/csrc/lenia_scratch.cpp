@@ -1,8 +1,8 @@
 // Allocation-free FFT-form stepping: lenia_fft_step_scratch.
 //
 // lenia_world_step runs on buffers a world allocates at create (state
-// spectrum F, column-pass output G, kernel spectra S, twiddle tables, tile
-// lists, the device record).  Callers that cannot allocate device memory
+// spectrum F, column-pass output G, kernel spectra S, twiddle tables, the
+// device record).  Callers that cannot allocate device memory
 // (code under stream capture, serving loops, harnesses that forbid hipMalloc)
 // get the same step on a caller-owned device field with every one of those
 // buffers carved out of one file-scope device array, `scratch`, of
@@ -45,21 +45,17 @@ struct DeviceScratch {
 std::mutex g_mu;
 DeviceScratch g_dev[kMaxDevices];
 
-// the record and tile lists of one call, written on the device from kernel
-// arguments (no host buffer has to outlive the call)
-__global__ void k_scratch_record(FftWorldDev rec, FftWorldDev *dst, int2 *rowTiles, int nRow,
-                                 int2 *colTiles, int nCol) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *dst = rec;
-    for (int i = t; i < nRow; i += gridDim.x * blockDim.x) rowTiles[i] = make_int2(0, i);
-    for (int i = t; i < nCol; i += gridDim.x * blockDim.x) colTiles[i] = make_int2(0, i);
+// the record of one call, written on the device from a kernel argument (no
+// host buffer has to outlive the call)
+__global__ void k_scratch_record(FftWorldDev rec, FftWorldDev *dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = rec;
 }
 
 size_t up(size_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
 
 // byte offsets of one call's buffers inside the array
 struct Layout {
-    size_t rec, clock, rowTiles, colTiles, twW, twH, raw, win, cw, F, G, S, total;
+    size_t rec, clock, twW, twH, raw, win, cw, F, G, S, total;
     int nmax;
 };
 
@@ -77,8 +73,6 @@ Layout layout_of(const lenia_world_desc &d) {
     };
     L.rec = take(sizeof(FftWorldDev));
     L.clock = take(sizeof(StepClock));
-    L.rowTiles = take(H / 2 * sizeof(int2));
-    L.colTiles = take(Wh * sizeof(int2));
     L.twW = take(W * sizeof(float2));
     L.twH = take(H * sizeof(float2));
     L.raw = take(std::max(W, H) * sizeof(float2));
@@ -148,8 +142,6 @@ extern "C" int lenia_fft_step_scratch(lenia_ctx *ctx, const lenia_world_desc *de
     hw.twH = reinterpret_cast<float *>(base + L.twH);
     hw.clock = reinterpret_cast<StepClock *>(base + L.clock);
     hw.devDesc = base + L.rec;
-    hw.rowTiles = reinterpret_cast<int2 *>(base + L.rowTiles);
-    hw.colTiles = reinterpret_cast<int2 *>(base + L.colTiles);
     const FftWorldDev rec = fft_record(&hw);
 
     hipError_t e = hipSuccess;
@@ -186,9 +178,8 @@ extern "C" int lenia_fft_step_scratch(lenia_ctx *ctx, const lenia_world_desc *de
         ds.key = *desc;
         ds.haveKey = true;
     }
-    hipLaunchKernelGGL(k_scratch_record, dim3(4), dim3(256), 0, s, rec,
-                       reinterpret_cast<FftWorldDev *>(hw.devDesc), hw.rowTiles, hw.nRowTiles,
-                       hw.colTiles, hw.nColTiles);
+    hipLaunchKernelGGL(k_scratch_record, dim3(1), dim3(64), 0, s, rec,
+                       reinterpret_cast<FftWorldDev *>(hw.devDesc));
     e = hipGetLastError();
     // step i runs at frame frameIndex + i, substep 0 (lenia_world_step's clock)
     if (e == hipSuccess) e = launch_set_clock(hw.clock, frameIndex, 0, 1, 0, s);
