@@ -359,9 +359,15 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     forward_rows<T>(Z, Zt, wd, tw, rp0, nt, tid);
 }
 
+// Waves per SIMD the register allocation must leave room for: the 256-thread
+// 12-cell step kernel (the batched sweep's row pass) fits 96 VGPRs without
+// spilling, which lets a CU hold five of its workgroups instead of four
+// (105 VGPRs unconstrained); at 80 VGPRs (six) it spills.
+constexpr int rows_waves_per_eu(int T, int V) { return T == 256 && V == kMidValues ? 5 : 1; }
+
 // One tile per workgroup (grid = tiles)
 template <int MODE, int T, bool EXTRAS, bool KPAR, int V = row_values(T)>
-__global__ __launch_bounds__(T) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(rows_waves_per_eu(T, V)))) void k_fft_rows(const FftWorldDev *__restrict__ worlds,
                                                 const int2 *__restrict__ tiles, int nTiles,
                                                 uint32_t stepOff, int kSel,
                                                 float *__restrict__ out) {
