@@ -465,12 +465,4 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * base + min(x, rem) + j;
 }
 
-// The tile a step workgroup works on: entry j of the launch's tile table, or,
-// for a launch without a table (one world, whose tiles are 0, 1, ... in order),
-// tile j of world 0.  Without the table the tile is known without a memory
-// round trip before the descriptor loads.
-__device__ __forceinline__ int2 step_tile(const int2 *__restrict__ tiles, int j) {
-    return tiles ? tiles[j] : make_int2(0, j);
-}
-
 }  // namespace lenia

@@ -70,8 +70,7 @@ struct lenia_world {
     lenia::FftWorldDev fdev{};
     lenia::DirectWorldDev ddev{};
     lenia::MfmaWorldDev mdev{};
-    int2 *rowTiles = nullptr, *colTiles = nullptr;
-    int nRowTiles = 0, nColTiles = 0;
+    int nRowTiles = 0, nColTiles = 0;  // tiles per pass (no table: step_tile)
     size_t ldsRows = 0, ldsCols = 0;
 
     uint32_t parity() const { return (uint32_t)(hostSteps & 1u); }

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
     const int half = SPLIT && tid >= kDHalf, ht = tid - half * kDHalf;  // window-row half, thread in it
     const int tx = ht & 7, ty = ht >> 3;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
-        const int2 tile = tiles[ti];
+        const int2 tile = step_tile(tiles, ti);
         const DirectWorldDev &wd = worlds[tile.x];
         const int W = wd.W, H = wd.H, h = wd.half, n = 2 * h + 1, np = wd.npad;
         const int x0 = (tile.y % wd.tilesX) * kDTX;

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
-        const int2 tile = tiles[ti];
+        const int2 tile = step_tile(tiles, ti);
         const MfmaWorldDev &wd = worlds[tile.x];
         const int W = wd.W, H = wd.H, h = wd.half;
         const int x0 = (tile.y % wd.tilesX) * (32 * NBX);

@@ -223,4 +223,12 @@ __device__ __forceinline__ void advance_clocks(const WorldT *worlds, int nWorlds
     }
 }
 
+// The tile a step workgroup works on: entry j of the launch's tile table, or,
+// for a launch without a table (one world, whose tiles are 0, 1, ... in order),
+// tile j of world 0.  Without the table the tile is known without a memory
+// round trip before the descriptor loads.
+__device__ __forceinline__ int2 step_tile(const int2 *__restrict__ tiles, int j) {
+    return tiles ? tiles[j] : make_int2(0, j);
+}
+
 }  // namespace lenia

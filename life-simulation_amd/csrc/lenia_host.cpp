@@ -799,11 +799,8 @@ int lenia::world_upload_desc(lenia_world *w) {
         w->mdev = md;
         HIP_TRY(hipMemcpyAsync(w->devDesc, &md, sizeof(md), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
-        std::vector<int2> t;
-        for (int i = 0; i < md.tilesX * md.tilesY; ++i) t.push_back(make_int2(0, i));
-        HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
-                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
-        w->nRowTiles = (int)t.size();
+        // no tile table: the kernels take tile j of world 0 (step_tile)
+        w->nRowTiles = md.tilesX * md.tilesY;
         w->ldsRows = mfma_lds_bytes(w->half, md.W, w->split());
     } else {
         DirectWorldDev dd;
@@ -826,11 +823,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         w->ddev = dd;
         HIP_TRY(hipMemcpyAsync(w->devDesc, &dd, sizeof(dd), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
-        std::vector<int2> t;
-        for (int i = 0; i < dd.tilesX * dd.tilesY; ++i) t.push_back(make_int2(0, i));
-        HIP_TRY(hipMemcpyAsync(w->rowTiles, t.data(), t.size() * sizeof(int2),
-                               hipMemcpyHostToDevice, ctx->stream), "upload tiles");
-        w->nRowTiles = (int)t.size();
+        w->nRowTiles = dd.tilesX * dd.tilesY;  // no tile table (step_tile)
         w->ldsRows = direct_lds_bytes(w->half);
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload world descriptor");
@@ -842,7 +835,7 @@ static void world_release(lenia_world *w) {
     world_drop_graph(w);
     world_free_adjoint(w);
     void *ptrs[] = {w->A0, w->A1, w->F, w->G, w->S, w->twW, w->twH, w->winF, w->toepHi, w->toepMid, w->toepLo,
-                    w->clock, w->devDesc, w->rowTiles, w->colTiles, w->scratchOut, w->org};
+                    w->clock, w->devDesc, w->scratchOut, w->org};
     for (void *p : ptrs) dev_free(ctx, p);
 }
 
@@ -897,17 +890,9 @@ extern "C" int lenia_world_create(lenia_ctx *ctx, const lenia_world_desc *desc, 
         if (!rc) rc = dev_alloc(ctx, (void **)&w->S, K * Wh * H * sizeof(float));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twW, (size_t)W * sizeof(float2));
         if (!rc) rc = dev_alloc(ctx, (void **)&w->twH, (size_t)H * sizeof(float2));
-        // no tile tables: the step kernels take tile j of world 0 from the
-        // workgroup id (step_tile)
-    } else {
-        int tx, ty;
-        if (w->isMfma)
-            mfma_tile_shape(w->split(), &tx, &ty);
-        else
-            direct_tile_shape(&tx, &ty);
-        int nt = ((W + tx - 1) / tx) * ((H + ty - 1) / ty);
-        if (!rc) rc = dev_alloc(ctx, (void **)&w->rowTiles, nt * sizeof(int2));
     }
+    // no tile tables: a single world's step kernels take tile j of world 0
+    // from the workgroup id (step_tile)
     if (rc) {
         world_release(w);
         delete w;
@@ -1082,11 +1067,11 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
                                   ctx->stream);
     } else if (w->isMfma) {
         if (ev) launch_events() = {ev[0], ev[1]};
-        e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+        e = launch_mfma(0, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, nullptr, w->nRowTiles,
                         w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream);
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
-        e = launch_direct(0, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+        e = launch_direct(0, (const DirectWorldDev *)w->devDesc, nullptr, w->nRowTiles,
                           w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream, ctx->numCUs);
     }
     launch_events() = LaunchEvents();
@@ -1302,10 +1287,10 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
             e = world_launch_rows(w, 2, (const FftWorldDev *)w->devDesc, 0, (int)k, w->scratchOut, false,
                                   ctx->stream);
     } else if (w->isMfma) {
-        e = launch_mfma(1, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+        e = launch_mfma(1, w->split(), w->ksteps, (const MfmaWorldDev *)w->devDesc, nullptr, w->nRowTiles,
                         w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
     } else {
-        e = launch_direct(1, (const DirectWorldDev *)w->devDesc, w->rowTiles, w->nRowTiles,
+        e = launch_direct(1, (const DirectWorldDev *)w->devDesc, nullptr, w->nRowTiles,
                           w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream, ctx->numCUs);
     }
     if (e != hipSuccess) return hip_fail(e, "potential launch");
