@@ -5,16 +5,17 @@
 // extended to up to four kernels with the leniaGrowthMulti rule
 // (Lenia.metal:161-232).
 //
-// A workgroup owns a 64 (x) x 32 (y) output tile and stages its (32+2h) x
-// (64+P) toroidal halo in LDS once (P = window width rounded up to 8; the
-// window rows are zero padded to P).  Each output's potential is the sum over
+// A workgroup owns a 64 (x) x TY (y) output tile, TY = 32, or 16 for a world
+// whose 32-row tiles would leave half the CUs without one, and stages its
+// (TY+2h) x (64+P) toroidal halo in LDS once (P = window width rounded up to
+// 8; the window rows are zero padded to P).  Each output's potential is the sum over
 // the first ceil(n/2) window rows plus the sum over the others.  In the
 // 256-thread form one thread produces 8 horizontally adjacent outputs of one
-// row, both halves in turn; in the 512-thread form (SPLIT, for grids of at
-// most one tile per CU) a thread of the first half of the
+// row, both halves in turn; in the split form (SPLIT, 16 TY threads, for
+// grids of at most one tile per CU) a thread of the first half of the
 // workgroup sums the first window-row half and its partner in the second
-// half the other, and the first adds the partner's sums (through LDS).  Both
-// forms compute the same bits.  For every window row it slides
+// half the other, and the first adds the partner's sums (through LDS).  Every
+// form and tile height computes the same bits.  For every window row it slides
 // a 16-value register window along the staged row, 8 taps at a time: two
 // 16-byte LDS reads of state and two of weights (the kernel's window is
 // staged in LDS too) feed 64 FMAs, with the next chunk's reads issued before
@@ -29,9 +30,9 @@
 
 namespace lenia {
 
-constexpr int kDHalf = 256;  // threads of one window-row half: 8 (x) x 32 (y)
-constexpr int kDTX = 64;   // output tile width
-constexpr int kDTY = 32;   // output tile height
+constexpr int kDTX = kDirectTileW;  // output tile width
+constexpr int kDTY = kDirectTileH;  // output tile height (the 16-row form: kDTYSmall)
+constexpr int kDTYSmall = 16;
 constexpr int kDOX = 8;    // outputs per thread (x)
 constexpr int kDStage = 16;
 
@@ -67,12 +68,14 @@ __device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const
 }
 
 // MODE 0: full step (src -> dst by clock parity); MODE 1: potential of kernel
-// kSel of the current state into `out`.  SPLIT: 512 threads, one window-row
-// half each; else 256 threads, both halves each.
-template <int MODE, bool SPLIT>
-__global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
+// kSel of the current state into `out`.  TY: output tile height; a window-row
+// half takes 8 TY threads (8 x TY outputs of 8).  SPLIT: 16 TY threads, one
+// window-row half each; else 8 TY threads, both halves each.
+template <int MODE, bool SPLIT, int TY>
+__global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
     const DirectWorldDev *__restrict__ worlds, const int2 *__restrict__ tiles, int nTiles, uint32_t stepOff,
     int kSel, float *__restrict__ out) {
+    constexpr int kDHalf = 8 * TY;  // threads of one window-row half
     constexpr int kDT = SPLIT ? 2 * kDHalf : kDHalf;
     extern __shared__ float tileS[];
     const int tid = threadIdx.x;
@@ -81,15 +84,28 @@ __global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
     for (int ti = blockIdx.x; ti < nTiles; ti += gridDim.x) {
         const int2 tile = step_tile(tiles, ti);
         const DirectWorldDev &wd = worlds[tile.x];
-        const int W = wd.W, H = wd.H, h = wd.half, n = 2 * h + 1, np = wd.npad;
+        const int W = wd.W, H = wd.H, h = wd.half, np = wd.npad;
         const int x0 = (tile.y % wd.tilesX) * kDTX;
-        const int y0 = (tile.y / wd.tilesX) * kDTY;
-        const int rows = kDTY + 2 * h, cols = kDTX + np, pitch = cols + 4;
+        const int y0 = (tile.y / wd.tilesX) * TY;
+        const int rows = TY + 2 * h, cols = kDTX + np, pitch = cols + 4;
         const StepClock ck = clock_at(wd.clock, stepOff);
         const LENIA_GLOBAL float *src = gp(ck.parity ? wd.A1 : wd.A0);
         LENIA_GLOBAL float *dst = gp(ck.parity ? wd.A0 : wd.A1);
-        // halo rows y0-h .. y0+31+h, columns x0-h .. x0-h+cols-1 (toroidal;
-        // columns past x0+63+h only meet zero weights)
+        // halo rows y0-h .. y0+TY-1+h, columns x0-h .. x0-h+cols-1 (toroidal;
+        // columns past x0+63+h only meet zero weights).  The wrapped row
+        // offsets and columns are tabulated in LDS first (one modulo per
+        // entry), so staging a value takes a table lookup, not two modulos
+        // and a division.
+        const int n = 2 * h + 1;
+        int *rowOff = reinterpret_cast<int *>(tileS + rows * pitch + n * np + 8 * kDHalf);  // [rows]
+        int *colIx = rowOff + rows;                                                          // [cols]
+        for (int i = tid; i < rows + cols; i += kDT) {
+            if (i < rows)
+                rowOff[i] = wrap_i(y0 - h + i, H) * W;
+            else
+                colIx[i - rows] = wrap_i(x0 - h + (i - rows), W);
+        }
+        __syncthreads();
         const int nst = rows * cols;
         for (int base = 0; base < nst; base += kDT * kDStage) {
             float v[kDStage];
@@ -99,8 +115,9 @@ __global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
                 const int i = base + u * kDT + tid;
                 at[u] = -1;
                 if (i < nst) {
-                    const int ly = i / cols, lx = i - ly * cols;
-                    v[u] = src[(size_t)wrap_i(y0 - h + ly, H) * W + wrap_i(x0 - h + lx, W)];
+                    // i / cols by the record's magic (exact: i < 2^16, cols < 2^8)
+                    const int ly = (int)__umulhi((uint32_t)i, wd.colsMagic), lx = i - ly * cols;
+                    v[u] = src[rowOff[ly] + colIx[lx]];
                     at[u] = ly * pitch + lx;
                 }
             }
@@ -121,7 +138,7 @@ __global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
         const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
         const int pitch4 = pitch / 4;
         float *winS = tileS + rows * pitch;  // the current kernel's window, [n][np]
-        float4 *part4 = reinterpret_cast<float4 *>(winS + n * np);  // second-half sums [2][256]
+        float4 *part4 = reinterpret_cast<float4 *>(winS + n * np);  // second-half sums [2][kDHalf]
         const int nh = (n + 1) / 2;
         for (int k = kBeg; k < kEnd; ++k) {
             // stage kernel k's window in LDS: every load of the tap loop is
@@ -231,42 +248,60 @@ __global__ __launch_bounds__(SPLIT ? 2 * kDHalf : kDHalf) void k_direct(
     }
 }
 
-template <bool SPLIT>
+template <bool SPLIT, int TY>
 static void direct_launch(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles, int grid,
                           size_t lds, uint32_t stepOff, int kSel, float *out, hipStream_t s) {
-    constexpr int T = SPLIT ? 2 * kDHalf : kDHalf;
+    constexpr int T = SPLIT ? 16 * TY : 8 * TY;
     if (mode == 0)
-        launch_timed(k_direct<0, SPLIT>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel, out);
+        launch_timed(k_direct<0, SPLIT, TY>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel,
+                     out);
     else
-        launch_timed(k_direct<1, SPLIT>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel, out);
+        launch_timed(k_direct<1, SPLIT, TY>, dim3(grid), dim3(T), lds, s, worlds, tiles, nTiles, stepOff, kSel,
+                     out);
 }
 
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
                          int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                         hipStream_t s, int numCUs) {
-    // grids that leave CUs without a workgroup take the 512-thread form
-    // (bit-identical); from one tile per CU on, the 256-thread form, which
-    // keeps two workgroups per CU (1280x720 R=32: 460 tiles, 113 against 124 us)
-    if (nTiles <= numCUs)
-        direct_launch<true>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
-    else
-        direct_launch<false>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+                         hipStream_t s, int numCUs, int tileH) {
+    // grids that leave CUs without a workgroup take the split form
+    // (bit-identical); from one tile per CU on, the one-thread-per-8-outputs
+    // form, which keeps two workgroups per CU (1280x720 R=32: 460 tiles, 113
+    // against 124 us)
+    if (tileH == kDTYSmall) {
+        if (nTiles <= numCUs)
+            direct_launch<true, kDTYSmall>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else
+            direct_launch<false, kDTYSmall>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+    } else if (tileH == kDTY) {
+        if (nTiles <= numCUs)
+            direct_launch<true, kDTY>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+        else
+            direct_launch<false, kDTY>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
-void direct_tile_shape(int *tx, int *ty) {
+// 64 x 32 tiles, or 64 x 16 where 32-row tiles would number at most half the
+// CUs (512x512: 128 tiles of 64 x 32 leave half the CUs idle; 256 of 64 x 16
+// give every CU one)
+void direct_tile_shape(int W, int H, int numCUs, int *tx, int *ty) {
     *tx = kDTX;
-    *ty = kDTY;
+    const long tiles32 = (long)((W + kDTX - 1) / kDTX) * ((H + kDTY - 1) / kDTY);
+    *ty = 2 * tiles32 <= numCUs ? kDTYSmall : kDTY;
 }
 
 // window row pitch (floats): 2h+1 rounded up to 8
 int direct_npad(int h) { return (2 * h + 1 + 7) & ~7; }
 
-size_t direct_lds_bytes(int h) {
+size_t direct_lds_bytes(int h, int tileH) {
     const int pitch = kDTX + direct_npad(h) + 4;
-    // the halo tile, one kernel window [2h+1][npad], the second half's sums
-    return ((size_t)(kDTY + 2 * h) * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float) +
-           (size_t)2 * kDHalf * sizeof(float4);
+    // the halo tile, one kernel window [2h+1][npad], the second half's sums,
+    // the wrapped row offsets and columns of the halo
+    const int rows = tileH + 2 * h, cols = kDTX + direct_npad(h);
+    return ((size_t)rows * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float) +
+           (size_t)2 * 8 * tileH * sizeof(float4) + (size_t)(rows + cols) * sizeof(int);
 }
 
 }  // namespace lenia

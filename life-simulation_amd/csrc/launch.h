@@ -70,14 +70,18 @@ int fft_max_values();
 int fft_row_values(int T);
 
 // direct path (k_direct.hip).  mode 0: step; 1: potential of kernel kSel.
-// grids of at most numCUs tiles run the 512-thread form (each output's two
-// window-row halves on two threads; the same bits)
+// tileH: the output tile height the worlds' records were tiled with (32, or
+// 16 for direct_tile_shape's small grids).  Grids of at most numCUs tiles run
+// the split form (each output's two window-row halves on two threads; the
+// same bits)
 hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *tiles, int nTiles,
                          int grid, size_t lds, uint32_t stepOff, int kSel, float *out,
-                         hipStream_t s, int numCUs);
-void direct_tile_shape(int *tx, int *ty);
+                         hipStream_t s, int numCUs, int tileH);
+// tile shape of a single W x H world on numCUs CUs (batches use 64 x 32)
+void direct_tile_shape(int W, int H, int numCUs, int *tx, int *ty);
+constexpr int kDirectTileW = 64, kDirectTileH = 32;
 int direct_npad(int h);
-size_t direct_lds_bytes(int h);
+size_t direct_lds_bytes(int h, int tileH);
 
 // MFMA direct forms (k_mfma.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,

@@ -256,9 +256,14 @@ static int batch_setup_direct(lenia_batch *b) {
     std::vector<int2> rt;
     for (int i = 0; i < n; ++i) {
         lenia_world *w = b->worlds[i];
-        table[i] = w->ddev;
-        for (int t = 0; t < w->ddev.tilesX * w->ddev.tilesY; ++t) rt.push_back(make_int2(i, t));
-        b->ldsRows = std::max(b->ldsRows, w->ldsRows);
+        // the batch tiles every world 64 x 32 (a world alone may use 64 x 16
+        // tiles; each output's arithmetic is the same in both)
+        DirectWorldDev d = w->ddev;
+        d.tilesY = (d.H + kDirectTileH - 1) / kDirectTileH;
+        d.tileH = kDirectTileH;
+        table[i] = d;
+        for (int t = 0; t < d.tilesX * d.tilesY; ++t) rt.push_back(make_int2(i, t));
+        b->ldsRows = std::max(b->ldsRows, direct_lds_bytes(d.half, kDirectTileH));
     }
     b->nRowTiles = (int)rt.size();
     int rc = dev_alloc(ctx, &b->devWorlds, sizeof(DirectWorldDev) * n);
@@ -366,7 +371,7 @@ static hipError_t batch_enqueue_step(lenia_batch *b, const hipEvent_t *ev, uint3
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)b->devWorlds, b->rowTiles, b->nRowTiles,
-                          b->nRowTiles, b->ldsRows, off, 0, nullptr, s, b->ctx->numCUs);
+                          b->nRowTiles, b->ldsRows, off, 0, nullptr, s, b->ctx->numCUs, kDirectTileH);
         launch_events() = LaunchEvents();
     }
     return e;

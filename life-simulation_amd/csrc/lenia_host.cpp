@@ -366,7 +366,7 @@ static int check_desc(const lenia_world_desc *d) {
         if (kd.core == LENIA_CORE_GAUSSIAN && !(kd.shellSigma > 0.0f))
             return fail(LENIA_ERR_INVALID_ARG, "kernel %u: shellSigma must be > 0", k);
         int h = kernel_half(kd);
-        if (d->convMode == LENIA_CONV_DIRECT && (h > 43 || direct_lds_bytes(h) > kFftLdsMax))
+        if (d->convMode == LENIA_CONV_DIRECT && (h > 43 || direct_lds_bytes(h, kDirectTileH) > kFftLdsMax))
             return fail(LENIA_ERR_UNSUPPORTED,
                         "kernel %u: the FP32 direct form supports window half-widths h <= 43 "
                         "(got %d); use the FFT form", k, h);
@@ -806,13 +806,15 @@ int lenia::world_upload_desc(lenia_world *w) {
         DirectWorldDev dd;
         std::memset(&dd, 0, sizeof(dd));
         int tx, ty;
-        direct_tile_shape(&tx, &ty);
+        direct_tile_shape((int)w->desc.width, (int)w->desc.height, ctx->numCUs, &tx, &ty);
         dd.W = w->desc.width;
         dd.H = w->desc.height;
         dd.half = w->half;
         dd.tilesX = (dd.W + tx - 1) / tx;
         dd.tilesY = (dd.H + ty - 1) / ty;
+        dd.tileH = ty;
         dd.npad = direct_npad(w->half);
+        dd.colsMagic = div_magic(tx + dd.npad);
         dd.g = w->g;
         dd.win = w->winF;
         dd.rowChunks = reinterpret_cast<const int2 *>(
@@ -824,7 +826,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         HIP_TRY(hipMemcpyAsync(w->devDesc, &dd, sizeof(dd), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
         w->nRowTiles = dd.tilesX * dd.tilesY;  // no tile table (step_tile)
-        w->ldsRows = direct_lds_bytes(w->half);
+        w->ldsRows = direct_lds_bytes(w->half, ty);
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload world descriptor");
     return LENIA_OK;
@@ -1072,7 +1074,7 @@ hipError_t lenia::enqueue_step(lenia_world *w, const hipEvent_t *ev, uint32_t of
     } else {
         if (ev) launch_events() = {ev[0], ev[1]};
         e = launch_direct(0, (const DirectWorldDev *)w->devDesc, nullptr, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream, ctx->numCUs);
+                          w->nRowTiles, w->ldsRows, off, 0, nullptr, ctx->stream, ctx->numCUs, w->ddev.tileH);
     }
     launch_events() = LaunchEvents();
     return e;
@@ -1291,7 +1293,8 @@ extern "C" int lenia_world_potential(lenia_world *w, uint32_t k, float *host, si
                         w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream);
     } else {
         e = launch_direct(1, (const DirectWorldDev *)w->devDesc, nullptr, w->nRowTiles,
-                          w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream, ctx->numCUs);
+                          w->nRowTiles, w->ldsRows, 0, (int)k, w->scratchOut, ctx->stream, ctx->numCUs,
+                          w->ddev.tileH);
     }
     if (e != hipSuccess) return hip_fail(e, "potential launch");
     HIP_TRY(hipMemcpyAsync(host, w->scratchOut, cells * sizeof(float), hipMemcpyDeviceToHost,

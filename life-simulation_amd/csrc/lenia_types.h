@@ -161,7 +161,8 @@ struct DirectWorldDev {
     int32_t half;       // window half-width (max over kernels)
     int32_t tilesX, tilesY;
     int32_t npad;       // window row pitch: 2 half + 1 rounded up to 8
-    int32_t pad0[2];
+    int32_t tileH;      // output tile height the tiles count in (32 or 16)
+    uint32_t colsMagic; // ceil(2^32 / (64 + npad)): halo index / columns by a high multiply
     GrowthParams g;
     const float *win;   // [K][2 half + 1][npad] float windows (zero padded)
     const int2 *rowChunks;  // [K][2 half + 1]: first and past-last 8-tap chunk with a non-zero weight
