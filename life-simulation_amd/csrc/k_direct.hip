@@ -54,9 +54,6 @@ __device__ __forceinline__ void lds_read8(const float4 *p, float (&v)[8]) {
     v[7] = b.w;
 }
 
-// per-row chunk ranges are the same for every lane: read them through the
-// constant address space, so they arrive as scalar loads
-#define LENIA_CONST __attribute__((address_space(4)))
 
 // 8 taps of one window row: out[i] += w[t] * in[i + t], in = lo ++ hi
 __device__ __forceinline__ void taps8(float (&rs)[8], const float (&w)[8], const float (&lo)[8],
@@ -97,8 +94,12 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
         // entry), so staging a value takes a table lookup, not two modulos
         // and a division.
         const int n = 2 * h + 1;
-        int *rowOff = reinterpret_cast<int *>(tileS + rows * pitch + n * np + 8 * kDHalf);  // [rows]
+        // (after the halo, the window and, in the split form, the second
+        // half's sums)
+        int *rowOff = reinterpret_cast<int *>(tileS + rows * pitch + n * np + (SPLIT ? 8 * kDHalf : 0));
         int *colIx = rowOff + rows;                                                          // [cols]
+        // the current kernel's per-row chunk ranges [n], 8-byte aligned
+        int2 *rcS = reinterpret_cast<int2 *>(rowOff + ((rows + cols + 1) & ~1));
         for (int i = tid; i < rows + cols; i += kDT) {
             if (i < rows)
                 rowOff[i] = wrap_i(y0 - h + i, H) * W;
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
         }
         __syncthreads();
         const int nst = rows * cols;
+        const uint32_t colsMagic = wd.colsMagic;  // one scalar load, not one per staged value
         for (int base = 0; base < nst; base += kDT * kDStage) {
             float v[kDStage];
             int at[kDStage];
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
                 at[u] = -1;
                 if (i < nst) {
                     // i / cols by the record's magic (exact: i < 2^16, cols < 2^8)
-                    const int ly = (int)__umulhi((uint32_t)i, wd.colsMagic), lx = i - ly * cols;
+                    const int ly = (int)__umulhi((uint32_t)i, colsMagic), lx = i - ly * cols;
                     v[u] = src[rowOff[ly] + colIx[lx]];
                     at[u] = ly * pitch + lx;
                 }
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
         const float4 *tile4 = reinterpret_cast<const float4 *>(tileS);
         const int pitch4 = pitch / 4;
         float *winS = tileS + rows * pitch;  // the current kernel's window, [n][np]
-        float4 *part4 = reinterpret_cast<float4 *>(winS + n * np);  // second-half sums [2][kDHalf]
+        float4 *part4 = reinterpret_cast<float4 *>(winS + n * np);  // SPLIT: second-half sums [2][kDHalf]
         const int nh = (n + 1) / 2;
         for (int k = kBeg; k < kEnd; ++k) {
             // stage kernel k's window in LDS: every load of the tap loop is
@@ -150,18 +152,27 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
                     gp(reinterpret_cast<const float4 *>(wd.win)) + (size_t)k * n * (np / 4);
                 float4 *ws4 = reinterpret_cast<float4 *>(winS);
                 for (int i = tid; i < n * (np / 4); i += kDT) ws4[i] = gld4(wg + i);
+                // its row chunk ranges too: read from LDS (in order with the
+                // tap loop's other LDS reads) a row's range needs no wait for
+                // every outstanding load, as a scalar load's would
+                const LENIA_GLOBAL int *rcg = gp(reinterpret_cast<const int *>(wd.rowChunks)) + (size_t)2 * k * n;
+                for (int i = tid; i < n; i += kDT) rcS[i] = make_int2(rcg[2 * i], rcg[2 * i + 1]);
             }
             __syncthreads();
-            const LENIA_CONST int *rc = (const LENIA_CONST int *)(wd.rowChunks) + (size_t)2 * k * n;
             // the sum of window rows iy0 .. iy1-1 into tot
             auto rows_sum = [&](int iy0, int iy1, float (&tot)[kDOX]) {
 #pragma unroll
                 for (int i = 0; i < kDOX; ++i) tot[i] = 0.0f;
+                // the next row's chunk range is read a row ahead
+                int2 rNext = rcS[iy0];
                 for (int iy = iy0; iy < iy1; ++iy) {
                     const float4 *trow = tile4 + (ty + iy) * pitch4 + 2 * tx;
                     const float4 *wrow = reinterpret_cast<const float4 *>(winS) + iy * (np / 4);
                     // chunks [c0, c1) of this row hold its non-zero weights
-                    const int c1 = rc[2 * iy + 1];
+                    // (the same for every lane)
+                    const int c0 = __builtin_amdgcn_readfirstlane(rNext.x);
+                    const int c1 = __builtin_amdgcn_readfirstlane(rNext.y);
+                    if (iy + 1 < iy1) rNext = rcS[iy + 1];
                     float rs[kDOX];
 #pragma unroll
                     for (int i = 0; i < kDOX; ++i) rs[i] = 0.0f;
@@ -171,7 +182,7 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
                     // taps of chunk c+1 (reads stay inside the row: chunk c+3 is
                     // loaded only when c+3 <= c1)
                     float va[8], vb[8], w0[8];
-                    int c = rc[2 * iy];
+                    int c = c0;
                     lds_read8(trow + 2 * c, va);
                     lds_read8(trow + 2 * c + 2, vb);
                     lds_read8(wrow + 2 * c, w0);
@@ -223,25 +234,37 @@ __global__ __launch_bounds__(SPLIT ? 16 * TY : 8 * TY) void k_direct(
             float tot[kDOX];
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) tot[i] = lo[i] + hi[i];
+            // kernel k's growth constants, read once for the 8 outputs (the
+            // arithmetic of cell_jitter and growth_term)
+            const float muK = g.mu[k], scK = g.bellScale[k], wK = g.weight[k], muJit = g.muJitter;
+            const uint32_t gmode = g.growthMode;
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) {
                 const int x = xb + i;
                 if (MODE == 1) {
                     if (x < W && y < H) gp(out)[(size_t)y * W + x] = tot[i];
                 } else if (x < W && y < H) {
-                    float muJ = cell_jitter(g, x, y, seed, ck.frame);
-                    gacc[i] += growth_term(g, k, tot[i], muJ);
+                    float muJ = 0.0f;
+                    if (muJit > 0.0f) muJ = (rand01(x, y, 401u + ck.frame % 613u, seed) - 0.5f) * muJit * 0.05f;
+                    const float b = bell_of(tot[i], muK + muJ, scK);
+                    gacc[i] += gmode == 1u ? wK * b : wK * (2.0f * b - 1.0f);
                 }
             }
         }
         if (MODE == 0 && !half) {
+            // the finish constants copied to registers once, and every
+            // output computed before the stores (no store between the uses
+            // of a constant, so none is read again)
+            const GrowthParams gl = g;
+            float res[kDOX];
+#pragma unroll
+            for (int i = 0; i < kDOX; ++i)
+                res[i] = growth_finish(gl, tileS[(ty + h) * pitch + kDOX * tx + i + h], gacc[i], xb + i, y, W, H,
+                                       seed, ck.frame);
 #pragma unroll
             for (int i = 0; i < kDOX; ++i) {
                 const int x = xb + i;
-                if (x < W && y < H) {
-                    float a = tileS[(ty + h) * pitch + kDOX * tx + i + h];
-                    dst[(size_t)y * W + x] = growth_finish(g, a, gacc[i], x, y, W, H, seed, ck.frame);
-                }
+                if (x < W && y < H) dst[(size_t)y * W + x] = res[i];
             }
         }
         __syncthreads();
@@ -268,12 +291,12 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
     // form, which keeps two workgroups per CU (1280x720 R=32: 460 tiles, 113
     // against 124 us)
     if (tileH == kDTYSmall) {
-        if (nTiles <= numCUs)
+        if (direct_split(nTiles, numCUs))
             direct_launch<true, kDTYSmall>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
         else
             direct_launch<false, kDTYSmall>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
     } else if (tileH == kDTY) {
-        if (nTiles <= numCUs)
+        if (direct_split(nTiles, numCUs))
             direct_launch<true, kDTY>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
         else
             direct_launch<false, kDTY>(mode, worlds, tiles, nTiles, grid, lds, stepOff, kSel, out, s);
@@ -295,13 +318,17 @@ void direct_tile_shape(int W, int H, int numCUs, int *tx, int *ty) {
 // window row pitch (floats): 2h+1 rounded up to 8
 int direct_npad(int h) { return (2 * h + 1 + 7) & ~7; }
 
-size_t direct_lds_bytes(int h, int tileH) {
+bool direct_split(int nTiles, int numCUs) { return nTiles <= numCUs; }
+
+size_t direct_lds_bytes(int h, int tileH, bool split) {
     const int pitch = kDTX + direct_npad(h) + 4;
     // the halo tile, one kernel window [2h+1][npad], the second half's sums,
     // the wrapped row offsets and columns of the halo
+    // and the current kernel's row chunk ranges
     const int rows = tileH + 2 * h, cols = kDTX + direct_npad(h);
     return ((size_t)rows * pitch + (size_t)(2 * h + 1) * direct_npad(h)) * sizeof(float) +
-           (size_t)2 * 8 * tileH * sizeof(float4) + (size_t)(rows + cols) * sizeof(int);
+           (split ? (size_t)2 * 8 * tileH * sizeof(float4) : 0) + (size_t)((rows + cols + 1) & ~1) * sizeof(int) +
+           (size_t)(2 * h + 1) * sizeof(int2);
 }
 
 }  // namespace lenia

@@ -81,7 +81,10 @@ hipError_t launch_direct(int mode, const DirectWorldDev *worlds, const int2 *til
 void direct_tile_shape(int W, int H, int numCUs, int *tx, int *ty);
 constexpr int kDirectTileW = 64, kDirectTileH = 32;
 int direct_npad(int h);
-size_t direct_lds_bytes(int h, int tileH);
+// the split form runs for grids of at most one tile per CU; its LDS holds the
+// second half's sums as well
+bool direct_split(int nTiles, int numCUs);
+size_t direct_lds_bytes(int h, int tileH, bool split);
 
 // MFMA direct forms (k_mfma.hip).  mode 0: step; 1: potential of kernel kSel.
 hipError_t launch_mfma(int mode, bool split, int ksteps, const MfmaWorldDev *worlds,

@@ -263,8 +263,12 @@ static int batch_setup_direct(lenia_batch *b) {
         d.tileH = kDirectTileH;
         table[i] = d;
         for (int t = 0; t < d.tilesX * d.tilesY; ++t) rt.push_back(make_int2(i, t));
-        b->ldsRows = std::max(b->ldsRows, direct_lds_bytes(d.half, kDirectTileH));
     }
+    // the split form, for batches of at most one tile per CU, holds the
+    // second half's sums in LDS as well
+    for (int i = 0; i < n; ++i)
+        b->ldsRows = std::max(b->ldsRows, direct_lds_bytes(table[i].half, kDirectTileH,
+                                                           direct_split((int)rt.size(), ctx->numCUs)));
     b->nRowTiles = (int)rt.size();
     int rc = dev_alloc(ctx, &b->devWorlds, sizeof(DirectWorldDev) * n);
     if (!rc) rc = dev_alloc(ctx, (void **)&b->rowTiles, sizeof(int2) * rt.size());

@@ -366,7 +366,7 @@ static int check_desc(const lenia_world_desc *d) {
         if (kd.core == LENIA_CORE_GAUSSIAN && !(kd.shellSigma > 0.0f))
             return fail(LENIA_ERR_INVALID_ARG, "kernel %u: shellSigma must be > 0", k);
         int h = kernel_half(kd);
-        if (d->convMode == LENIA_CONV_DIRECT && (h > 43 || direct_lds_bytes(h, kDirectTileH) > kFftLdsMax))
+        if (d->convMode == LENIA_CONV_DIRECT && (h > 43 || direct_lds_bytes(h, kDirectTileH, true) > kFftLdsMax))
             return fail(LENIA_ERR_UNSUPPORTED,
                         "kernel %u: the FP32 direct form supports window half-widths h <= 43 "
                         "(got %d); use the FFT form", k, h);
@@ -826,7 +826,7 @@ int lenia::world_upload_desc(lenia_world *w) {
         HIP_TRY(hipMemcpyAsync(w->devDesc, &dd, sizeof(dd), hipMemcpyHostToDevice, ctx->stream),
                 "upload world descriptor");
         w->nRowTiles = dd.tilesX * dd.tilesY;  // no tile table (step_tile)
-        w->ldsRows = direct_lds_bytes(w->half, ty);
+        w->ldsRows = direct_lds_bytes(w->half, ty, direct_split(w->nRowTiles, ctx->numCUs));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream), "upload world descriptor");
     return LENIA_OK;
