@@ -386,20 +386,43 @@ __device__ __forceinline__ void wave_rows_split(const FftWorldDev &wd, int2 tile
 
 // --------------------------------------------------------------- column pass
 // Tile: C = NW M columns; wave w, group g runs column tile.y C + w M + g.
+// The record fields a column tile reads, requested together
+// with the plan code before the kernel's plan switch: the column's loads then
+// wait for one scalar round trip instead of two (the code, then the fields
+// inside the case).
+struct ColsFields {
+    const float *S, *F, *twH;
+    float *G, *saveAhat;
+    int Wh, C, K;
+};
+
+__device__ __forceinline__ ColsFields cols_fields(const FftWorldDev &wd) {
+    ColsFields f;
+    f.S = wd.S;
+    f.F = wd.F;
+    f.twH = wd.twH;
+    f.G = wd.G;
+    f.saveAhat = wd.saveAhat;
+    f.Wh = wd.Wh;
+    f.C = wd.C;
+    f.K = (int)wd.g.nKernels;
+    return f;
+}
+
 template <int N, int L, int... RS>
-__device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, float2 *tw,
+__device__ __forceinline__ void wave_cols(const ColsFields &cf, int2 tile, float2 *tw,
                                           float2 *xbw, int wave, int lane, int nThreads) {
     constexpr int V = N / L, M = 64 / L, H = N;
     const int g = lane / L, l = lane % L;
     float2 *xb = xbw + g * wave_group_span<N>();
-    const int Wh = wd.Wh;
-    const int c = tile.y * wd.C + wave * M + g;
+    const int Wh = cf.Wh;
+    const int c = tile.y * cf.C + wave * M + g;
     const bool live = c < Wh;
     const int cc = live ? c : Wh - 1;
     const int off = cc * H + l;  // (k, column) offsets fit 32 bits (K Wh H < 2^31)
     const int sStride = Wh * H;
-    const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + off;
-    const LENIA_GLOBAL float *S = gp(wd.S) + off;
+    const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(cf.F)) + off;
+    const LENIA_GLOBAL float *S = gp(cf.S) + off;
     float2 x[V];
     float sv[V];  // the next kernel's spectrum, requested before each transform
 #pragma unroll
@@ -407,15 +430,15 @@ __device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, floa
 #pragma unroll
     for (int m = 0; m < V; ++m) sv[m] = S[L * m];
     // the twiddle table, its loads in flight together with the column's
-    wave_load_twiddles(tw, wd.twH, H, nThreads);
+    wave_load_twiddles(tw, cf.twH, H, nThreads);
     wave_fft<N, L, false, 1, 0, RS...>(x, xb, tw, l);
-    if (wd.saveAhat && live) {
-        LENIA_GLOBAL float2 *sva = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + off;
+    if (cf.saveAhat && live) {
+        LENIA_GLOBAL float2 *sva = gp(reinterpret_cast<float2 *>(cf.saveAhat)) + off;
 #pragma unroll
         for (int m = 0; m < V; ++m) gst2(sva + L * m, x[m]);
     }
-    LENIA_GLOBAL float2 *G = gp(reinterpret_cast<float2 *>(wd.G)) + off;
-    const int K = (int)wd.g.nKernels;
+    LENIA_GLOBAL float2 *G = gp(reinterpret_cast<float2 *>(cf.G)) + off;
+    const int K = cf.K;
 #pragma unroll 1
     for (int k = 0; k < K; ++k) {
         float2 y[V];
@@ -438,38 +461,38 @@ __device__ __forceinline__ void wave_cols(const FftWorldDev &wd, int2 tile, floa
 // one barrier wave k < K multiplies by kernel k's spectrum, transforms back
 // and stores G_k.
 template <int N, int L, int... RS>
-__device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile, float2 *tw,
+__device__ __forceinline__ void wave_cols_split(const ColsFields &cf, int2 tile, float2 *tw,
                                                 float2 *xbw, float2 *xs, int wave, int lane) {
     constexpr int V = N / L, H = N;
     const int g = lane / L, l = lane % L;
     float2 *xb = xbw + g * wave_group_span<N>();
-    const int Wh = wd.Wh;
-    const int c = tile.y * wd.C + g;
+    const int Wh = cf.Wh;
+    const int c = tile.y * cf.C + g;
     const bool live = c < Wh;
     const int cc = live ? c : Wh - 1;
     const int off = cc * H + l;
     const int sStride = Wh * H;
-    const int K = (int)wd.g.nKernels;
+    const int K = cf.K;
     float2 x[V];
     float sv[V];
     if (wave < K) {
-        const LENIA_GLOBAL float *S = gp(wd.S) + off + wave * sStride;
+        const LENIA_GLOBAL float *S = gp(cf.S) + off + wave * sStride;
 #pragma unroll
         for (int m = 0; m < V; ++m) sv[m] = S[L * m];
     }
     if (wave == 0) {
-        const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + off;
+        const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(cf.F)) + off;
 #pragma unroll
         for (int m = 0; m < V; ++m) x[m] = gld2(F + L * m);
     }
     // the twiddle table, its loads in flight together with the column's
-    wave_load_twiddles(tw, wd.twH, H, 64 * kMaxKernels);
+    wave_load_twiddles(tw, cf.twH, H, 64 * kMaxKernels);
     if (wave == 0) {
         wave_fft<N, L, false, 1, 0, RS...>(x, xb, tw, l);
 #pragma unroll
         for (int m = 0; m < V; ++m) xs[m * 64 + lane] = x[m];
-        if (wd.saveAhat && live) {
-            LENIA_GLOBAL float2 *sva = gp(reinterpret_cast<float2 *>(wd.saveAhat)) + off;
+        if (cf.saveAhat && live) {
+            LENIA_GLOBAL float2 *sva = gp(reinterpret_cast<float2 *>(cf.saveAhat)) + off;
 #pragma unroll
             for (int m = 0; m < V; ++m) gst2(sva + L * m, x[m]);
         }
@@ -481,7 +504,7 @@ __device__ __forceinline__ void wave_cols_split(const FftWorldDev &wd, int2 tile
     for (int m = 0; m < V; ++m) y[m] = cscale(xs[m * 64 + lane], sv[m]);
     wave_fft<N, L, true, 1, 0, RS...>(y, xb, tw, l);
     if (live) {
-        LENIA_GLOBAL float2 *G = gp(reinterpret_cast<float2 *>(wd.G)) + off + wave * sStride;
+        LENIA_GLOBAL float2 *G = gp(reinterpret_cast<float2 *>(cf.G)) + off + wave * sStride;
 #pragma unroll
         for (int m = 0; m < V; ++m) gst2(G + L * m, y[m]);
     }
@@ -541,11 +564,12 @@ __global__ __launch_bounds__(64 * NW) void k_wave_cols(const FftWorldDev *__rest
     const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    const ColsFields cf = cols_fields(wd);
     switch (wd.planH.code) {
 #define LENIA_WAVE_COLS_CASE(code, n, L, ...)                                                   \
     case code:                                                                                   \
         if constexpr (n / L == VC)                                                               \
-            wave_cols<n, L, __VA_ARGS__>(wd, tile, smem,                                        \
+            wave_cols<n, L, __VA_ARGS__>(cf, tile, smem,                                        \
                                          smem + n + wave * (64 / L) * wave_group_span<n>(), wave, lane, \
                                          64 * NW);                                               \
         break;
@@ -590,13 +614,14 @@ __global__ __launch_bounds__(64 * kMaxKernels) void k_wave_cols_split(
     const int2 tile = step_tile(tiles, xcd_tile((int)blockIdx.x, nTiles));
     const FftWorldDev &wd = worlds[tile.x];
     const int wave = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63u);
+    const ColsFields cf = cols_fields(wd);
     switch (wd.planH.code) {
 #define LENIA_WAVE_COLS_SPLIT_CASE(code, n, L, ...)                                        \
     case code:                                                                              \
         if constexpr (n / L == VC) {                                                        \
             float2 *xbw = smem + n + wave * (64 / L) * wave_group_span<n>();                \
             float2 *xs = smem + n + kMaxKernels * (64 / L) * wave_group_span<n>();          \
-            wave_cols_split<n, L, __VA_ARGS__>(wd, tile, smem, xbw, xs, wave, lane);        \
+            wave_cols_split<n, L, __VA_ARGS__>(cf, tile, smem, xbw, xs, wave, lane);        \
         }                                                                                   \
         break;
         LENIA_WAVE_PLANS(LENIA_WAVE_COLS_SPLIT_CASE)
