@@ -169,7 +169,11 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     const int W = wd.W, H = wd.H, Wh = wd.Wh, P = wd.P, H2 = H / 2;
     const int rp0 = tile.y * P;
     const int nt = min(P, H2 - rp0);
-    const uint32_t mP = wd.mP;
+    // magics read once: a field read inside an unrolled per-value loop is
+    // read again in every iteration, each time after a full wait (the serial
+    // forms read mW where they use it: held, it let the compiler keep every
+    // value's indices live across their kernel loop, 276 -> 349 VGPRs)
+    const uint32_t mP = wd.mP, mW = wd.mW;
     float2 *tw = smem;
     float2 *Z = smem + W;
     // the twin transform region, when the launch has one (rows_twin)
@@ -177,14 +181,26 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     LENIA_GLOBAL float *Arow = gp(wd.A) + (size_t)(2 * rp0) * W;  // first row of this tile
     const int nq = nt * W;
     const int nm = P * Wh;  // (kx, p) slots, p fastest; p >= nt are idle
+    // the gather's operands are read here, ahead of the twiddle copy, so the
+    // scalar loads of the tile's prologue go out together
+    const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
+    const size_t gStride = (size_t)Wh * H2;
+    const int nK = (int)wd.g.nKernels;
     const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twW));
+    // an empty statement that takes these operands in scalar registers: their
+    // scalar loads are issued together here and waited for once, where the
+    // compiler would otherwise issue each next to its first use, one wait
+    // after another before the gather starts (side-by-side step kernels only:
+    // in the serial form it raised the registers a thread holds)
+    if constexpr (KPAR && MODE == 1)
+        asm volatile("" ::"s"(wd.G), "s"(wd.A), "s"(wd.twW), "s"(W), "s"(P), "s"(nK), "s"(mW), "s"(mP));
     TwCopy<T> twc;
     if (newWorld) twc.issue(twg, W, tid);
 
     if (MODE == 0) {
         if (newWorld) twc.store(tw, twg, W, tid);
         for (int q = tid; q < nq; q += T) {
-            int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+            int p = (int)fast_div((uint32_t)q, KPAR ? mW : wd.mW), x = q - p * W;
             int ia = 2 * p * W + x;
             Z[sw(q)] = make_float2(Arow[ia], Arow[ia + W]);
         }
@@ -198,15 +214,13 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     LENIA_GLOBAL float *const saveA = gp(wd.saveA);
     const StepClock ck = clock_at(wd.clock, stepOff);
     const uint32_t seed = clock_seed(ck, g.moduleSeed);
-    const LENIA_GLOBAL float4 *G4 = gp(reinterpret_cast<const float4 *>(wd.G)) + rp0;
-    const size_t gStride = (size_t)Wh * H2;
     float ar0[kV], ar1[kV];
     if (MODE == 1) {
 #pragma unroll
         for (int i = 0; i < kV; ++i) {
             int q = tid + i * T;
             if (q < nq) {
-                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int p = (int)fast_div((uint32_t)q, KPAR ? mW : wd.mW), x = q - p * W;
                 int ia = 2 * p * W + x;
                 ar0[i] = Arow[ia];
                 ar1[i] = Arow[ia + W];
@@ -215,7 +229,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
     }
     if (KPAR || MODE == 2) {
         const int kBeg = MODE == 2 ? kSel : 0;
-        const int kn = MODE == 2 ? 1 : (int)g.nKernels;
+        const int kn = MODE == 2 ? 1 : nK;
         float4 gv[kG];
         int zi[kG], kxv[kG];
         gather_issue<T, kG>(gv, zi, kxv, G4, gStride, kBeg, kn, H2, nt, P, mP, nm, W, tid);
@@ -226,7 +240,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         const float2 *U = fft_run<true, kMaxV, T>(Z, wd.planW, kn * nt, tw, tid, Zt, true);
         if (MODE == 2) {
             for (int q = tid; q < nq; q += T) {
-                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int p = (int)fast_div((uint32_t)q, mW), x = q - p * W;
                 int ia = 2 * p * W + x;
                 float2 z = U[sw(q)];
                 LENIA_GLOBAL float *o = gp(out) + (size_t)(2 * rp0) * W;
@@ -239,7 +253,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         for (int i = 0; i < kV; ++i) {
             int q = tid + i * T;
             if (q < nq) {
-                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int p = (int)fast_div((uint32_t)q, mW), x = q - p * W;
                 int ya = 2 * (rp0 + p);
                 float ja = 0.0f, jb = 0.0f;
                 if (EXTRAS) {
@@ -282,7 +296,7 @@ __device__ __forceinline__ void rows_tile(const FftWorldDev *__restrict__ worlds
         for (int i = 0; i < kV; ++i) {
             int q = tid + i * T;
             if (q < nq) {
-                int p = (int)fast_div((uint32_t)q, wd.mW), x = q - p * W;
+                int p = (int)fast_div((uint32_t)q, mW), x = q - p * W;
                 int ia = 2 * p * W + x;
                 Arow[ia] = ar0[i];
                 Arow[ia + W] = ar1[i];
