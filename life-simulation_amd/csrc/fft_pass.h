@@ -409,6 +409,10 @@ __device__ __forceinline__ void cols_tile(const FftWorldDev *__restrict__ worlds
     float2 *tw = smem;
     float2 *X = smem + H;
     const LENIA_GLOBAL float2 *twg = gp(reinterpret_cast<const float2 *>(wd.twH));
+    // the operands of the column and spectrum loads, loaded together and
+    // waited for once (see rows_tile)
+    if constexpr (KPAR)
+        asm volatile("" ::"s"(wd.F), "s"(wd.S), "s"(wd.G), "s"(wd.twH), "s"(H), "s"(Wh), "s"(wd.C), "s"(K));
     TwCopy<T> twc;
     if (newWorld) twc.issue(twg, H, tid);
     const LENIA_GLOBAL float2 *F = gp(reinterpret_cast<const float2 *>(wd.F)) + (size_t)c0 * H;
