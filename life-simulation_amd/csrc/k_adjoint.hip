@@ -158,6 +158,9 @@ __global__ __launch_bounds__(T) void k_adj_mid(AdjArgs a, const float *U, const 
     const LENIA_GLOBAL float *Ug = gp(U);
     LENIA_GLOBAL float *gpo = gp(gpb);
     const size_t n = (size_t)W * H;
+    // the operands of the first loads (cells, twiddles, the later step's
+    // spectrum) in scalar registers: loaded together, waited for once
+    asm volatile("" ::"s"(U), "s"(A), "s"(g), "s"(G1), "s"(a.twW), "s"(W), "s"(H), "s"(K));
     // a cell pair's operands (rows 2rp and 2rp+1 at column x): potentials,
     // state, incoming gradient (EDGE: the gp part of it)
     auto load_x = [&](int x, float2 (&u2)[4], float2 &a2, float2 &g2) {
@@ -275,6 +278,11 @@ __global__ __launch_bounds__(T) void k_adj_cols(AdjArgs a, const float *Fin, con
     const LENIA_GLOBAL float2 *Ah = gp(reinterpret_cast<const float2 *>(Ahat)) + (size_t)c * H;
     const LENIA_GLOBAL float *Dc = gp(a.D) + (size_t)c * H;
     const LENIA_GLOBAL float *Sc = gp(a.S) + (size_t)c * H;
+    // an empty statement taking the operands of the first loads in scalar
+    // registers: their kernel-argument loads are issued together and waited
+    // for once, before the loads they address
+    asm volatile("" ::"s"(Ahat), "s"(Fin), "s"(a.D), "s"(a.S), "s"(a.twH), "s"(a.planH.code), "s"(nb), "s"(K),
+                 "s"(H), "s"(Wh));
     // per spectrum row ky: the state spectrum, the shell-gradient spectra and
     // the kernel spectra at (c, ky)
     auto load_y = [&](int ky, float2 &ah, float (&d)[16], float (&sk)[4]) {
