@@ -342,23 +342,32 @@ __global__ __launch_bounds__(kMT) void k_mfma(const MfmaWorldDev *__restrict__ w
             }
             __syncthreads();  // the next kernel's partials reuse the buffer
             if (wave < NB) {
+                // kernel k's growth constants, read once for the 16 outputs
+                // (the arithmetic of cell_jitter and growth_term): a field
+                // read per output is read again each time, after a full wait
+                const float muK = g.mu[k], scK = g.bellScale[k], wK = g.weight[k], muJit = g.muJitter;
+                const uint32_t gmode = g.growthMode;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int x = xq + 8 * (r >> 2) + 4 * grp + (r & 3);
                     if (MODE == 1) {
                         if (x < W && yb < H) gp(out)[(size_t)yb * W + x] = mine[r];
                     } else if (x < W && yb < H) {
-                        float muJ = cell_jitter(g, x, yb, seed, ck.frame);
-                        gacc[r] += growth_term(g, k, mine[r], muJ);
+                        float muJ = 0.0f;
+                        if (muJit > 0.0f)
+                            muJ = (rand01(x, yb, 401u + ck.frame % 613u, seed) - 0.5f) * muJit * 0.05f;
+                        const float bl = bell_of(mine[r], muK + muJ, scK);
+                        gacc[r] += gmode == 1u ? wK * bl : wK * (2.0f * bl - 1.0f);
                     }
                 }
             }
         }
         if (MODE == 0 && wave < NB) {
+            const GrowthParams gl = g;  // the finish constants in registers, read once
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int x = xq + 8 * (r >> 2) + 4 * grp + (r & 3);
-                aCur[r] = growth_finish(g, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
+                aCur[r] = growth_finish(gl, aCur[r], gacc[r], x, yb, W, H, seed, ck.frame);
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
