@@ -208,7 +208,9 @@ int lenia_world_autotune(lenia_world *w, uint32_t nSteps, double *msPerStep);
  * height (column pass) on the FFT form of desc, launch shapes tuned per plan,
  * nSteps steps each; records the fastest pair in the cache and returns it
  * (packed as fftPlanW / fftPlanH; any output may be NULL) with its kernel ms
- * per step.  Worlds that already exist keep their plans. */
+ * per step.  A column plan that runs on the wave executor is preferred when
+ * it is within 10% of the fastest (batches run such worlds' column passes in
+ * their wave class launches).  Worlds that already exist keep their plans. */
 int lenia_plan_autotune(lenia_ctx *ctx, const lenia_world_desc *desc, uint32_t nSteps,
                         uint32_t *planW, uint32_t *planH, double *msPerStep);
 /* one entry: axis 0 rows, 1 columns; plan 0 removes it; a plan that is not a

@@ -309,14 +309,37 @@ extern "C" int lenia_plan_autotune(lenia_ctx *ctx, const lenia_world_desc *desc,
     }
     if (!bestW) return fail(LENIA_ERR_UNSUPPORTED, "lenia_plan_autotune: no radix plan of width %d runs", W);
     bestMs = 1e30;
+    // column plans with a wave kernel, and their times
+    auto has_wave = [&](uint32_t packed) {
+        FftPlan p;
+        return plan_unpack(H, packed, p) && wave_lanes(p.code) > 0;
+    };
+    double bestWaveMs = 1e30;
+    uint32_t bestWaveH = 0;
     for (uint32_t ph : plan_candidates(H)) {
         double ms;
-        if (time_plans(ctx, *desc, bestW, ph, nSteps, ms) && ms < bestMs) {
+        if (!time_plans(ctx, *desc, bestW, ph, nSteps, ms)) continue;
+        if (ms < bestMs) {
             bestMs = ms;
             bestH = ph;
         }
+        if (has_wave(ph) && ms < bestWaveMs) {
+            bestWaveMs = ms;
+            bestWaveH = ph;
+        }
     }
     if (!bestH) return fail(LENIA_ERR_UNSUPPORTED, "lenia_plan_autotune: no radix plan of height %d runs", H);
+    // A column plan with a wave kernel within kWaveColsSlack of the fastest is
+    // taken: in a batch the worlds whose column plan has none form an LDS
+    // launch of their own beside the wave class launches, which can set the
+    // column pass's length (the 256-world sweep averaged ~153 us per step
+    // with 384 = 8 4 4 3 against ~159 us with 16 8 3, the plan that was
+    // faster for a 384-high world alone).
+    constexpr double kWaveColsSlack = 1.10;
+    if (bestWaveH && !has_wave(bestH) && bestWaveMs <= bestMs * kWaveColsSlack) {
+        bestH = bestWaveH;
+        bestMs = bestWaveMs;
+    }
     int rc = lenia_plan_cache_set(0u, (uint32_t)W, bestW);
     if (!rc) rc = lenia_plan_cache_set(1u, (uint32_t)H, bestH);
     if (rc) return rc;

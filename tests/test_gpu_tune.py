@@ -172,7 +172,8 @@ def test_compiled_plans_match_oracle(ctx, n):
 
 def test_plan_autotune_shared_by_worlds_and_batches(ctx):
     """lenia_plan_autotune records the fastest compiled plans of a world's
-    width and height; worlds created afterwards take them, a batch of such
+    width and height (for the height, a plan with a wave kernel within 10% of
+    the fastest); worlds created afterwards take them, a batch of such
     worlds equals the same worlds stepped alone bit for bit, and the step is
     within 1e-5 of the FP64 oracle.  An explicit plan still wins."""
     import lenia
